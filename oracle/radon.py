@@ -1,0 +1,118 @@
+"""ctypes bindings of the C restatement in ``radon_oracle.c`` (test infrastructure only).
+
+The wrappers reproduce what the reference's Cython modules do around the C call:
+
+* ``radon_distance_driven.pyx:14-62``: degrees -> radians when ``max(angles) > 2*pi``,
+  ``np.fft.fftshift`` of the angles, a zeroed float32 output, ``radon_dd(...)``.
+* ``radon_ray_driven.pyx:15-61``: the same without the fftshift.
+
+Unlike the reference wrappers they never modify the caller's angle array in place.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+IEEE = 0   # the C as written, IEEE float32, no contraction
+FAST = 1   # the C as gcc -O3 -ffast-math compiles it (the reference's own flags)
+F64 = 2    # double-precision restatement
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "_build", "liboracle.so")
+_lib = None
+
+_f32p = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+_f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+
+
+def build():
+    """Compile ``liboracle.so`` with the committed Makefile."""
+    import subprocess
+    subprocess.run(["make", "-C", _HERE, "_build/liboracle.so"], check=True,
+                   stdout=subprocess.DEVNULL)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = ctypes.CDLL(_LIB_PATH)
+        for name in ("oracle_radon_dd", "oracle_radon_rd"):
+            f = getattr(L, name)
+            f.argtypes = [_f32p, ctypes.c_int, _f32p, ctypes.c_int, ctypes.c_int, _f32p,
+                          ctypes.c_int]
+            f.restype = ctypes.c_int
+        for name in ("oracle_radon_dd_f64", "oracle_radon_rd_f64"):
+            f = getattr(L, name)
+            f.argtypes = [_f64p, ctypes.c_int, _f64p, ctypes.c_int, ctypes.c_int, _f64p]
+            f.restype = ctypes.c_int
+        _lib = L
+    return _lib
+
+
+def _prep_angles(angles, shift):
+    a = np.array(angles, dtype=np.float32, copy=True).reshape(-1)
+    if a.size and np.max(a) > 2 * np.pi:
+        a *= np.float32(np.pi / 180.0)  # in-place float32 multiply, as the .pyx does
+    if shift:
+        a = np.fft.fftshift(a)
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def _call(kind, x, angles, oper, mode, shift):
+    L = lib()
+    npix = x.shape[1] if oper == 0 else x.shape[1]
+    a32 = _prep_angles(angles, shift)
+    nang = a32.size
+    if mode == F64:
+        a = a32.astype(np.float64)
+        fn = getattr(L, "oracle_radon_%s_f64" % kind)
+        if oper == 0:
+            img = np.ascontiguousarray(x, dtype=np.float64)
+            out = np.zeros((nang, npix), dtype=np.float64)
+            rc = fn(img, npix, a, nang, 0, out)
+        else:
+            sino = np.ascontiguousarray(x, dtype=np.float64)
+            out = np.zeros((npix, npix), dtype=np.float64)
+            rc = fn(out, npix, a, nang, 1, sino)
+    else:
+        fn = getattr(L, "oracle_radon_%s" % kind)
+        if oper == 0:
+            img = np.ascontiguousarray(x, dtype=np.float32)
+            out = np.zeros((nang, npix), dtype=np.float32)
+            rc = fn(img, npix, a32, nang, 0, out, mode)
+        else:
+            sino = np.ascontiguousarray(x, dtype=np.float32)
+            out = np.zeros((npix, npix), dtype=np.float32)
+            rc = fn(out, npix, a32, nang, 1, sino, mode)
+    if rc != 0:
+        raise MemoryError("oracle_radon_%s failed" % kind)
+    return out
+
+
+def dd_forward(image, angles, mode=FAST):
+    """Distance-driven forward projection, (N,N) -> (V,N), sinogram rows in the
+    reference's order (fftshift + reversal)."""
+    return _call("dd", image, angles, 0, mode, shift=True)
+
+
+def dd_back(sino, angles, mode=FAST):
+    return _call("dd", sino, angles, 1, mode, shift=True)
+
+
+def rd_forward(image, angles, mode=FAST):
+    """Ray-driven forward projection, (N,N) -> (V,N)."""
+    return _call("rd", image, angles, 0, mode, shift=False)
+
+
+def rd_back(sino, angles, mode=FAST):
+    return _call("rd", sino, angles, 1, mode, shift=False)
+
+
+def forward(oper, image, angles, mode=FAST):
+    return {"dd": dd_forward, "rd": rd_forward}[oper](image, angles, mode)
+
+
+def back(oper, sino, angles, mode=FAST):
+    return {"dd": dd_back, "rd": rd_back}[oper](sino, angles, mode)

@@ -1,0 +1,85 @@
+"""ctypes bindings of the reference's own radon C (test infrastructure only).
+
+``oracle/Makefile`` (``make ref``) compiles the reference's
+``common/operators/radon_module/*/radon_{dd,rd,pd,ss}.c`` where they lie into
+``oracle/_ref/libradon_<proj>_<flags>.so`` with ``<flags>`` in ``fast`` (the reference's
+own ``-O3 -ffast-math`` flags) and ``strict`` (IEEE).  Those libraries are git-ignored;
+when they are absent every function here raises :class:`FileNotFoundError` and the
+tests that need them skip.
+
+The C signatures bound here are ``radon_dd.c:54`` / ``radon_rd.c:76``
+(``void f(float* image, int npix, float* angles, int nang, int oper, float* sino)``)
+and ``radon_pd.c:20`` / ``radon_ss.c:93`` (an extra ``int method`` before ``sino``).
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+REF_DIR = os.path.join(_HERE, "_ref")
+_libs = {}
+_f32p = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+
+
+def path(proj, flags):
+    return os.path.join(REF_DIR, "libradon_%s_%s.so" % (proj, flags))
+
+
+def available(proj="dd", flags="fast"):
+    return os.path.exists(path(proj, flags))
+
+
+def _fn(proj, flags):
+    key = (proj, flags)
+    if key not in _libs:
+        p = path(proj, flags)
+        if not os.path.exists(p):
+            raise FileNotFoundError(p)
+        L = ctypes.CDLL(p)
+        f = getattr(L, "radon_" + proj)
+        if proj in ("pd", "ss"):
+            f.argtypes = [_f32p, ctypes.c_int, _f32p, ctypes.c_int, ctypes.c_int,
+                          ctypes.c_int, _f32p]
+        else:
+            f.argtypes = [_f32p, ctypes.c_int, _f32p, ctypes.c_int, ctypes.c_int, _f32p]
+        f.restype = None
+        _libs[key] = f
+    return _libs[key]
+
+
+def _angles(angles, proj):
+    a = np.array(angles, dtype=np.float32, copy=True).reshape(-1)
+    if a.size and np.max(a) > 2 * np.pi:
+        # radon_distance_driven.pyx:25-26 (pd/ss multiply by 2*pi/180 instead:
+        # radon_pixel_driven.pyx:27-28)
+        a *= np.float32((2 * np.pi if proj in ("pd", "ss") else np.pi) / 180.0)
+    if proj in ("dd", "pd", "ss"):
+        a = np.fft.fftshift(a)
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def forward(proj, image, angles, flags="fast", method=1):
+    f = _fn(proj, flags)
+    img = np.ascontiguousarray(image, dtype=np.float32)
+    npix = img.shape[0]
+    a = _angles(angles, proj)
+    sino = np.zeros((a.size, npix), dtype=np.float32)
+    if proj in ("pd", "ss"):
+        f(img, npix, a, a.size, 0, method, sino)
+    else:
+        f(img, npix, a, a.size, 0, sino)
+    return sino
+
+
+def back(proj, sino, angles, flags="fast", method=1):
+    f = _fn(proj, flags)
+    s = np.ascontiguousarray(sino, dtype=np.float32)
+    nang, npix = s.shape
+    a = _angles(angles, proj)
+    image = np.zeros((npix, npix), dtype=np.float32)
+    if proj in ("pd", "ss"):
+        f(image, npix, a, nang, 1, method, s)
+    else:
+        f(image, npix, a, nang, 1, s)
+    return image
