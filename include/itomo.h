@@ -1,0 +1,145 @@
+/*
+ * itomo -- MI355X (gfx950) parallel-beam projectors and the iterations built on them.
+ *
+ * C ABI of libitomo.so.  Every pointer named image/sino/x/b/R/C/work is a DEVICE
+ * pointer (HBM, float32, C-contiguous) unless the comment says "host".  `stream` is a
+ * hipStream_t passed as void* (NULL = the null stream).  Calls are stream-ordered and
+ * asynchronous unless stated; they allocate no device memory unless stated.
+ *
+ * Array layouts are the reference's:
+ *   image : npix x npix, row-major (img[row][col])
+ *   sino  : nang x npix; for the distance-driven projector row nang-1-v holds view v of
+ *           the fftshift-ed angle list (radon_dd.c:79, radon_distance_driven.pyx:27);
+ *           for the ray-driven projector row v holds view v (radon_rd.c:90).
+ * Batched calls take `nslices` images (stride npix*npix) and sinograms (stride nang*npix)
+ * back to back.
+ *
+ * Status: 0 = ITOMO_OK, negative = error (itomo_last_error() gives the detail).
+ */
+#ifndef ITOMO_H
+#define ITOMO_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ITOMO_OK 0
+#define ITOMO_EINVAL (-1)     /* bad argument (size, pointer, flag) */
+#define ITOMO_EANGLE (-2)     /* an angle the reference C cannot handle (see plan) */
+#define ITOMO_EHIP (-3)       /* HIP runtime error */
+#define ITOMO_ENOMEM (-4)     /* host allocation failed */
+
+#define ITOMO_PROJ_DD 0       /* distance-driven, radon_dd.c */
+#define ITOMO_PROJ_RD 1       /* ray-driven, radon_rd.c */
+
+#define ITOMO_ARITH_FAST 0    /* float arithmetic of the reference as built with its own
+                                 flags (-O3 -ffast-math, create_module.py:14) */
+#define ITOMO_ARITH_IEEE 1    /* float arithmetic of the reference C as written (IEEE) */
+
+typedef struct itomo_plan itomo_plan;
+
+/* Library version string and the detail of the last error on this thread. */
+const char *itomo_version(void);
+const char *itomo_last_error(void);
+
+/* ---------------------------------------------------------------------------------
+ * Reference-signature entry points.
+ * Replace `void radon_dd(float* image, int npix, float* angles, int nang, int oper,
+ * float* sino)` (radon_dd.c:54) and `radon_rd` (radon_rd.c:76), which the Cython
+ * modules bind at radon_distance_driven.pyx:8 and radon_ray_driven.pyx:8.
+ * Same arguments and meaning: `angles` (HOST, radians, as the C receives them, i.e.
+ * already fftshift-ed for dd), oper 0 = forward (accumulates into sino), 1 = back
+ * (accumulates into image).  Adds a stream and a status return.  Synchronous with
+ * respect to the host for the plan upload only; the projection itself is queued on
+ * `stream`.  Allocates (and frees) a small per-call geometry table.
+ * ------------------------------------------------------------------------------- */
+int itomo_radon_dd(float *image, int npix, const float *angles, int nang, int oper,
+                   float *sino, void *stream);
+int itomo_radon_rd(float *image, int npix, const float *angles, int nang, int oper,
+                   float *sino, void *stream);
+
+/* ---------------------------------------------------------------------------------
+ * Plans: per-geometry constants (per-view sin/cos in the reference's float
+ * arithmetic, branch, output row) uploaded once.  `angles` is HOST memory, radians,
+ * in the order the reference C receives them.
+ * Distance-driven plans reject (ITOMO_EANGLE) views that sweep rows with sin <= 0 or
+ * columns with cos <= 0: for those the reference's sweep indexes bin -1 (undefined
+ * behaviour in radon_dd.c:158-188).  Angles in [0, pi) are always accepted.
+ * Replaces class_projectors_radon.projectors.__init__ (class_projectors_radon.py:47-55).
+ * ------------------------------------------------------------------------------- */
+int itomo_plan_create(itomo_plan **plan, int projector, int npix, const float *angles,
+                      int nang, int arith);
+int itomo_plan_destroy(itomo_plan *plan);
+/* Host copy of the per-view sin/cos the plan uses (for checking). */
+int itomo_plan_trig(const itomo_plan *plan, float *s_host, float *c_host);
+
+/* Forward projection A (class_projectors_radon.py:60-68): sino = A image.
+ * accumulate = 0 overwrites sino, 1 adds to it (the reference C always adds into the
+ * zeroed array its wrapper allocates).  Deterministic. */
+int itomo_forward(const itomo_plan *plan, const float *image, float *sino, int nslices,
+                  int accumulate, void *stream);
+/* Back projection A^T (class_projectors_radon.py:73-81): image = A^T sino. */
+int itomo_back(const itomo_plan *plan, const float *sino, float *image, int nslices,
+               int accumulate, void *stream);
+
+/* ---------------------------------------------------------------------------------
+ * SIRT (algorithms/art.py:211-237):
+ *   R = 1 / A(1) and C = 1 / A^T(1), zero entries replaced by 1e-10 before inverting.
+ *   x <- x + C * A^T( R * (b - A x) ), then, when pc_beta >= 0, x[x < 0] *= pc_beta
+ *   (utils.phys_constr, art.py:314-321).
+ * R: nang x npix, C: npix x npix (one geometry, shared by all slices).
+ * work: nslices x nang x npix scratch (receives R*(b-Ax)).
+ * stats (DEVICE, double[3*nslices] or NULL): per slice
+ *   [0] sum (b - A x_k)^2  (x_k = the iterate entering this step)
+ *   [1] sum (x_{k+1} - x_k)^2
+ *   [2] sum x_k^2
+ * all accumulated in FP64 in a fixed order (bitwise reproducible).
+ * ------------------------------------------------------------------------------- */
+int itomo_sirt_weights(const itomo_plan *plan, float *R, float *C, void *stream);
+/* The two halves of one step (the iterate call is forward then back):
+ *   forward: work = R * (b - A x), stats[0]
+ *   back:    x = x + C * A^T work (+ non-negativity), stats[1], stats[2] */
+int itomo_sirt_forward(const itomo_plan *plan, const float *x, const float *b,
+                       const float *R, float *work, int nslices, double *stats,
+                       void *stream);
+int itomo_sirt_back(const itomo_plan *plan, const float *work, const float *C, float *x,
+                    int nslices, float pc_beta, double *stats, void *stream);
+int itomo_sirt_iterate(const itomo_plan *plan, float *x, const float *b, const float *R,
+                       const float *C, float *work, int nslices, float pc_beta,
+                       double *stats, void *stream);
+
+/* ---------------------------------------------------------------------------------
+ * MLEM (algorithms/sir.py:204-283):
+ *   inv_d = 1 / d, d = A^T(1) with zeros replaced by 1e-10 (sir.py:215-217, 283).
+ *   Ax = A x; Ax[Ax == 0] += 2.220446e-16 (np.spacing(1)); q = b / Ax
+ *   x <- x * (inv_d * A^T q)
+ * work: nslices x nang x npix scratch (receives q).
+ * stats (DEVICE, double[3*nslices] or NULL): [0] sum (Ax - b)^2 (after the zero fix),
+ *   [1] sum (x_{k+1} - x_k)^2, [2] sum x_k^2.
+ * ------------------------------------------------------------------------------- */
+int itomo_mlem_weights(const itomo_plan *plan, float *inv_d, void *stream);
+/* halves of one step: forward work = b / A x (stats[0]); back x *= inv_d * A^T work */
+int itomo_mlem_forward(const itomo_plan *plan, const float *x, const float *b, float *work,
+                       int nslices, double *stats, void *stream);
+int itomo_mlem_back(const itomo_plan *plan, const float *work, const float *inv_d, float *x,
+                    int nslices, double *stats, void *stream);
+int itomo_mlem_iterate(const itomo_plan *plan, float *x, const float *b,
+                       const float *inv_d, float *work, int nslices, double *stats,
+                       void *stream);
+
+/* ---------------------------------------------------------------------------------
+ * FP64 reductions used by the monitoring / CG / ADMM / SPS scalars.
+ * out (DEVICE double[1]) = sum_i a[i]*b[i] over n float32 entries, accumulated in FP64
+ * in a fixed order.  b may equal a.  work: DEVICE scratch of ITOMO_DOT_WORK doubles.
+ * ------------------------------------------------------------------------------- */
+#define ITOMO_DOT_WORK 3072
+int itomo_dot_f64(const float *a, const float *b, int64_t n, double *out, double *work,
+                  void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ITOMO_H */

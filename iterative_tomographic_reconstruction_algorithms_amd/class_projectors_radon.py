@@ -1,0 +1,72 @@
+"""Drop-in for the reference's projector facade
+``algorithms/class_projectors_radon.py:44-110`` (class ``projectors``), running the
+radon projector pair on the GPU.
+
+Same constructor and attributes (``nang``, ``filt``, ``plot``, ``angles`` = float32
+radians, ``oper``).  ``A``/``At`` accept numpy arrays (returned as numpy) or float32 GPU
+tensors (kept on the GPU, no host round trip).  The geometry is uploaded once per
+object, lazily, so objects stay picklable for reference-style ``multiprocessing`` use.
+"""
+import numpy as np
+import torch
+
+from . import _dropin as D
+from .core import Plan
+
+myfloat = np.float32
+
+
+class projectors:
+
+    def __init__(self, npix, angles, oper="pd", ctr=0.0):
+        nang = len(angles)
+        angles1 = np.asarray(angles) * np.pi / 180.0       # class_projectors_radon.py:49
+        self.nang = nang
+        self.filt = "ramp"
+        self.plot = True
+        self.angles = angles1.astype(myfloat)              # class_projectors_radon.py:54
+        self.oper = oper
+        self.npix = int(npix)
+        self._plan = None
+
+    def __getstate__(self):
+        st = dict(self.__dict__)
+        st["_plan"] = None
+        return st
+
+    # C-order angles: dd (and pd/ss) fftshift (radon_distance_driven.pyx:27), rd not.
+    def _c_angles(self):
+        a = self.angles.copy()
+        if a.size and np.max(a) > 2 * np.pi:                # radon_*.pyx:25-26
+            a *= np.float32(np.pi / 180.0)
+        return np.fft.fftshift(a) if self.oper == "dd" else a
+
+    def plan(self, npix=None):
+        if self.oper not in ("dd", "rd"):
+            raise NotImplementedError(
+                "projector %r has no GPU implementation yet (available: 'dd', 'rd')"
+                % self.oper)
+        npix = self.npix if npix is None else int(npix)
+        if self._plan is None or self._plan.npix != npix:
+            self._plan = Plan(self.oper, npix, self._c_angles())
+        return self._plan
+
+    @staticmethod
+    def _as_f32(x):
+        if isinstance(x, torch.Tensor):
+            return x.to(torch.float32).contiguous()
+        return np.ascontiguousarray(x, dtype=myfloat)
+
+    # Forward projector (class_projectors_radon.py:60-68)
+    def A(self, x):
+        x = self._as_f32(x)
+        p = self.plan(x.shape[-1])
+        t, was_np = D.to_device(x)
+        return D.from_device(p.forward(t), was_np)
+
+    # Backprojector (class_projectors_radon.py:73-81)
+    def At(self, x):
+        x = self._as_f32(x)
+        p = self.plan(x.shape[-1])
+        t, was_np = D.to_device(x)
+        return D.from_device(p.back(t), was_np)

@@ -1,0 +1,134 @@
+// Projector epilogues: what a projector kernel does with the value it accumulated for
+// one output element, plus deterministic FP64 partial sums for the iteration monitors.
+#pragma once
+
+#include "itomo_internal.h"
+
+namespace itomo {
+
+enum Epi : int {
+    EPI_STORE = 0,      // out = acc (or out += acc when accumulating)
+    EPI_RECIP = 1,      // out = 1 / (acc == 0 ? acc + 1e-10f : acc)   (art.py:227-237)
+    EPI_SIRT_FWD = 2,   // out = R * (b - acc); part0 += (b - acc)^2          (art.py:212)
+    EPI_SIRT_BACK = 3,  // x = x + C * acc; x<0 -> x*beta; part1 += dx^2; part2 += x^2
+    EPI_MLEM_FWD = 4,   // ax = acc (0 -> 2^-52); out = b / ax; part0 += (ax - b)^2 (sir.py:270-271)
+    EPI_MLEM_BACK = 5,  // x = x * (inv_d * acc); part1 += dx^2; part2 += x^2     (sir.py:283)
+};
+
+struct EpiArgs {
+    const float* b;      // measured sinogram (slice stride = sino stride) -- forward epilogues
+    const float* w;      // R (nang x npix) / C or inv_d (npix x npix), shared by slices
+    float pc_beta;       // SIRT back: < 0 disables the non-negativity step
+    double* partials;    // [slice][block][3] FP64 partial sums, or nullptr
+    int blocks_per_slice;
+};
+
+// Block-wide deterministic sum of three doubles; thread 0 of the block writes them.
+// Requires blockDim.x*blockDim.y*blockDim.z a multiple of 64 and at most 1024.
+__device__ __forceinline__ void block_partials3(double a0, double a1, double a2,
+                                                double* dst)
+{
+    __shared__ double red[3][16];
+    for (int off = 32; off > 0; off >>= 1) {
+        a0 += __shfl_down(a0, off, 64);
+        a1 += __shfl_down(a1, off, 64);
+        a2 += __shfl_down(a2, off, 64);
+    }
+    const int tid = threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z);
+    const int nthreads = blockDim.x * blockDim.y * blockDim.z;
+    const int lane = tid & 63, wid = tid >> 6, nw = nthreads >> 6;
+    if (lane == 0) {
+        red[0][wid] = a0;
+        red[1][wid] = a1;
+        red[2][wid] = a2;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+        for (int k = 0; k < nw; ++k) {
+            s0 += red[0][k];
+            s1 += red[1][k];
+            s2 += red[2][k];
+        }
+        dst[0] = s0;
+        dst[1] = s1;
+        dst[2] = s2;
+    }
+}
+
+// Forward-side epilogue for output element index `o` (within one slice) of slice z.
+// Returns the value to store; writes the residual term to *p0.
+template <int EPI>
+__device__ __forceinline__ float epi_forward(float acc, int64_t o, int64_t slice_off,
+                                             const EpiArgs& e, double* p0)
+{
+    if (EPI == EPI_SIRT_FWD) {
+        const float bv = e.b[slice_off + o];
+        const float dif = __fsub_rn(bv, acc);
+        *p0 = (double)dif * (double)dif;
+        return __fmul_rn(e.w[o], dif);
+    } else if (EPI == EPI_MLEM_FWD) {
+        const float bv = e.b[slice_off + o];
+        float ax = acc;
+        if (ax == 0.0f) ax = __fadd_rn(ax, 2.220446049250313e-16f);
+        const double dd = (double)ax - (double)bv;
+        *p0 = dd * dd;
+        return __fdiv_rn(bv, ax);
+    } else if (EPI == EPI_RECIP) {
+        float v = acc;
+        if (v == 0.0f) v = __fadd_rn(v, 1e-10f);
+        return __fdiv_rn(1.0f, v);
+    }
+    return acc;
+}
+
+// Back-side epilogue for pixel index `o` of slice z: `xo` is the current iterate value.
+template <int EPI>
+__device__ __forceinline__ float epi_back(float acc, float xo, int64_t o, const EpiArgs& e,
+                                          double* p1, double* p2)
+{
+    if (EPI == EPI_SIRT_BACK) {
+        float xn = __fadd_rn(xo, __fmul_rn(e.w[o], acc));
+        if (e.pc_beta >= 0.0f && xn < 0.0f) xn = __fmul_rn(xn, e.pc_beta);
+        const double dd = (double)xn - (double)xo;
+        *p1 = dd * dd;
+        *p2 = (double)xo * (double)xo;
+        return xn;
+    } else if (EPI == EPI_MLEM_BACK) {
+        const float xn = __fmul_rn(xo, __fmul_rn(e.w[o], acc));
+        const double dd = (double)xn - (double)xo;
+        *p1 = dd * dd;
+        *p2 = (double)xo * (double)xo;
+        return xn;
+    } else if (EPI == EPI_RECIP) {
+        float v = acc;
+        if (v == 0.0f) v = __fadd_rn(v, 1e-10f);
+        return __fdiv_rn(1.0f, v);
+    }
+    return acc;
+}
+
+constexpr bool epi_has_partials(int epi)
+{
+    return epi == EPI_SIRT_FWD || epi == EPI_SIRT_BACK || epi == EPI_MLEM_FWD ||
+           epi == EPI_MLEM_BACK;
+}
+
+// host helpers (iter.hip)
+int fill(float* dst, float value, int64_t n, hipStream_t st);
+int finalize_partials(const double* partials, int blocks_per_slice, int nslices,
+                      int which_from, int which_to, double* stats, hipStream_t st);
+int elementwise_back_epilogue(int epi, const float* acc_img, float* x, const EpiArgs& e,
+                              int64_t npix2, int nslices, hipStream_t st);
+int elementwise_fwd_epilogue(int epi, float* sino_io, const EpiArgs& e, int64_t nsino,
+                             int nslices, hipStream_t st);
+
+// projector launchers with epilogues (dd.hip / rd.hip)
+int dd_forward_epi(const Plan& p, int epi, int ones_input, const float* image, float* out,
+                   int nslices, int accumulate, const EpiArgs& e, hipStream_t st,
+                   int* blocks_per_slice);
+int dd_back_epi(const Plan& p, int epi, int ones_input, const float* sino, float* out,
+                int nslices, int accumulate, const EpiArgs& e, hipStream_t st,
+                int* blocks_per_slice);
+
+}  // namespace itomo

@@ -1,0 +1,85 @@
+// Internal declarations shared by the HIP translation units of libitomo.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+#include "itomo.h"
+
+namespace itomo {
+
+// Per-view geometry kinds of the distance-driven projector (radon_dd.c:75-283).
+enum DdKind : int {
+    DD_NONE = 0,   // no branch taken (theta == 0 with |sin| >= 1e-7, or NaN)
+    DD_ROWSUM = 1, // |sin| < 1e-7: sino[row][i] += sum_j img[i][j]
+    DD_COLSUM = 2, // |cos| < 1e-7: sino[row][i] += sum_j img[j][i]
+    DD_SWEEP_ROWS = 3, // theta > pi/2: sweep along image rows
+    DD_SWEEP_COLS = 4, // 0 < theta < pi/2: sweep along image columns
+};
+
+// Per-view geometry kinds of the ray-driven projector (radon_rd.c:96-187).
+enum RdKind : int {
+    RD_COLSUM = 1,  // |sin| < 1e-7: sino[v][N-1-i] += sum_j img[j][i], i >= 1
+    RD_ROWSUM = 2,  // |cos| < 1e-7: sino[v][i] += sum_j img[i][j]
+    RD_RAYS = 3,    // Siddon traversal of 5 sub-rays per bin
+};
+
+// Constants of one distance-driven view, computed on the host in float exactly as
+// the reference's compiled C computes them.
+//   line coordinate  u_i = (i - nh) + 0.5
+//   pixel boundaries P_j = (j - nh) - K_i      (j = 0..N)
+//   bin boundaries   D_d = (d - nh) * rinv     (FAST) or (d - nh) / b (IEEE)
+//   K_i = u_i * t (FAST) or (u_i * a) / b (IEEE)
+struct DdView {
+    int kind;
+    int row;     // output sinogram row (nang-1-v)
+    float rinv;  // 1/sin (rows) or 1/cos (columns)
+    float t;     // cos*rinv (rows) or sin*rinv (columns)
+    float a, b;  // IEEE form: (cos, sin) for rows, (sin, cos) for columns
+    float est;   // b, used only to estimate a bin index (result is corrected exactly)
+    float s, c;  // sin, cos of the view as the reference computes them
+    float pad[2];
+};
+
+struct RdView {
+    int kind;
+    int lt_half_pi;  // theta < pi/2 (double compare, radon_rd.c:125-128)
+    float theta, s, c;
+    float rc, rs;    // 1/c, 1/s (FAST form)
+    float pad;
+};
+
+struct Plan {
+    int projector;   // ITOMO_PROJ_*
+    int npix;
+    int nang;
+    int arith;       // ITOMO_ARITH_*
+    void* d_geom;    // device array of DdView / RdView
+    void* h_geom;    // host copy
+    // Device scratch owned by the plan (FP64 partial sums, ray-driven back-projection
+    // image).  Grown on first use; a plan must not be used from two streams at once.
+    mutable void* scratch;
+    mutable size_t scratch_bytes;
+};
+
+// Returns plan scratch of at least `bytes` bytes (allocating on first use / growth).
+int plan_scratch(const Plan& p, size_t bytes, void** ptr, hipStream_t st);
+
+// error reporting
+int set_error(int status, const char* fmt, ...);
+int check_hip(hipError_t e, const char* what);
+
+// launchers (dd.hip / rd.hip / iter.hip)
+int dd_forward(const Plan& p, const float* image, float* sino, int nslices,
+               int64_t img_stride, int64_t sino_stride, int accumulate, hipStream_t st);
+int dd_back(const Plan& p, const float* sino, float* image, int nslices,
+            int64_t sino_stride, int64_t img_stride, int accumulate, hipStream_t st);
+int rd_forward(const Plan& p, const float* image, float* sino, int nslices,
+               int64_t img_stride, int64_t sino_stride, int accumulate, hipStream_t st);
+int rd_back(const Plan& p, const float* sino, float* image, int nslices,
+            int64_t sino_stride, int64_t img_stride, int accumulate, hipStream_t st);
+
+}  // namespace itomo
+
+// The opaque handle of the C ABI is the plan itself.
+struct itomo_plan : itomo::Plan {};

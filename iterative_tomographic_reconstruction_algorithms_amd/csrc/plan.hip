@@ -1,0 +1,292 @@
+// Host side of libitomo: error reporting, geometry plans and the C-ABI dispatch.
+//
+// Per-view constants are computed here, on the host, in the float arithmetic of the
+// reference C as its own build compiles it (gcc -O3 -ffast-math: sincosf for the
+// trigonometry, reciprocals hoisted out of the loops), or as written (IEEE), so the
+// device kernels only multiply, subtract and fuse -- never divide -- in FAST mode.
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "itomo_internal.h"
+
+namespace itomo {
+
+static thread_local char g_err[512] = "";
+
+int set_error(int status, const char* fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof g_err, fmt, ap);
+    va_end(ap);
+    return status;
+}
+
+int check_hip(hipError_t e, const char* what)
+{
+    if (e == hipSuccess) return ITOMO_OK;
+    return set_error(ITOMO_EHIP, "%s: %s", what, hipGetErrorString(e));
+}
+
+static const double kPi = 3.141592653589793;  // radon_dd.c:13
+static const double kEps = 1.0e-7;            // radon_dd.c:14
+
+static void view_trig(float theta, int arith, float* s, float* c)
+{
+    if (arith == ITOMO_ARITH_FAST) {
+        sincosf(theta, s, c);
+    } else {
+        *s = (float)std::sin((double)theta);
+        *c = (float)std::cos((double)theta);
+    }
+}
+
+// Branch selection and constants of radon_dd.c:69-283 for one view.
+static int make_dd_view(float theta, int arith, int row, DdView* g)
+{
+    float s, c;
+    view_trig(theta, arith, &s, &c);
+    std::memset(g, 0, sizeof *g);
+    g->row = row;
+    g->s = s;
+    g->c = c;
+    if (std::fabs((double)s) < kEps) {
+        g->kind = DD_ROWSUM;
+    } else if (std::fabs((double)c) < kEps) {
+        g->kind = DD_COLSUM;
+    } else if ((double)theta > kPi / 2) {
+        if (!(s > 0.0f))
+            return set_error(ITOMO_EANGLE,
+                             "dd: angle %.9g rad sweeps rows with sin <= 0; the reference "
+                             "sweep is undefined there (use angles in [0, pi))",
+                             (double)theta);
+        g->kind = DD_SWEEP_ROWS;
+        g->rinv = 1.0f / s;
+        g->t = c * g->rinv;
+        g->a = c;
+        g->b = s;
+        g->est = s;
+    } else if ((double)theta < kPi / 2.0 && theta != 0.0f) {
+        if (!(c > 0.0f))
+            return set_error(ITOMO_EANGLE,
+                             "dd: angle %.9g rad sweeps columns with cos <= 0; the reference "
+                             "sweep is undefined there (use angles in [0, pi))",
+                             (double)theta);
+        g->kind = DD_SWEEP_COLS;
+        g->rinv = 1.0f / c;
+        g->t = s * g->rinv;
+        g->a = s;
+        g->b = c;
+        g->est = c;
+    } else {
+        g->kind = DD_NONE;
+    }
+    return ITOMO_OK;
+}
+
+static int make_rd_view(float theta, int arith, RdView* g)
+{
+    float s, c;
+    view_trig(theta, arith, &s, &c);
+    std::memset(g, 0, sizeof *g);
+    g->theta = theta;
+    g->s = s;
+    g->c = c;
+    g->lt_half_pi = ((double)theta < kPi / 2) ? 1 : 0;
+    if (std::fabs((double)s) < kEps) {
+        g->kind = RD_COLSUM;
+    } else if (std::fabs((double)c) < kEps) {
+        g->kind = RD_ROWSUM;
+    } else {
+        g->kind = RD_RAYS;
+        g->rc = 1.0f / c;
+        g->rs = 1.0f / s;
+    }
+    return ITOMO_OK;
+}
+
+int plan_scratch(const Plan& p, size_t bytes, void** ptr, hipStream_t st)
+{
+    if (bytes > p.scratch_bytes) {
+        // queued work may still read the old buffer
+        int rc = check_hip(hipStreamSynchronize(st), "plan scratch: sync");
+        if (rc) return rc;
+        if (p.scratch) {
+            rc = check_hip(hipFree(p.scratch), "plan scratch: hipFree");
+            p.scratch = nullptr;
+            p.scratch_bytes = 0;
+            if (rc) return rc;
+        }
+        const size_t want = (bytes + 4095) & ~(size_t)4095;
+        rc = check_hip(hipMalloc(&p.scratch, want), "plan scratch: hipMalloc");
+        if (rc) return rc;
+        p.scratch_bytes = want;
+    }
+    *ptr = p.scratch;
+    return ITOMO_OK;
+}
+
+}  // namespace itomo
+
+using namespace itomo;
+
+extern "C" {
+
+const char* itomo_version(void) { return "itomo 0.1.0 (gfx950)"; }
+const char* itomo_last_error(void) { return g_err; }
+
+int itomo_plan_create(itomo_plan** out, int projector, int npix, const float* angles,
+                      int nang, int arith)
+{
+    if (!out) return set_error(ITOMO_EINVAL, "plan_create: null output pointer");
+    *out = nullptr;
+    if (projector != ITOMO_PROJ_DD && projector != ITOMO_PROJ_RD)
+        return set_error(ITOMO_EINVAL, "plan_create: unknown projector %d", projector);
+    if (arith != ITOMO_ARITH_FAST && arith != ITOMO_ARITH_IEEE)
+        return set_error(ITOMO_EINVAL, "plan_create: unknown arithmetic %d", arith);
+    // short indices in radon_dd.c:19-20 bound npix to 32766; keep well inside int math
+    if (npix < 1 || npix > 32766)
+        return set_error(ITOMO_EINVAL, "plan_create: npix %d outside [1, 32766]", npix);
+    if (nang < 1 || !angles)
+        return set_error(ITOMO_EINVAL, "plan_create: need nang >= 1 host angles");
+    for (int v = 0; v < nang; ++v)
+        if (!std::isfinite(angles[v]))
+            return set_error(ITOMO_EANGLE, "plan_create: angle %d is not finite", v);
+
+    itomo_plan* p = new (std::nothrow) itomo_plan();
+    if (!p) return set_error(ITOMO_ENOMEM, "plan_create: out of host memory");
+    p->projector = projector;
+    p->npix = npix;
+    p->nang = nang;
+    p->arith = arith;
+    size_t bytes = 0;
+    int st = ITOMO_OK;
+    if (projector == ITOMO_PROJ_DD) {
+        bytes = sizeof(DdView) * (size_t)nang;
+        DdView* h = (DdView*)std::malloc(bytes);
+        if (!h) { delete p; return set_error(ITOMO_ENOMEM, "plan_create: out of host memory"); }
+        for (int v = 0; v < nang && st == ITOMO_OK; ++v)
+            st = make_dd_view(angles[v], arith, nang - 1 - v, &h[v]);
+        p->h_geom = h;
+    } else {
+        bytes = sizeof(RdView) * (size_t)nang;
+        RdView* h = (RdView*)std::malloc(bytes);
+        if (!h) { delete p; return set_error(ITOMO_ENOMEM, "plan_create: out of host memory"); }
+        for (int v = 0; v < nang && st == ITOMO_OK; ++v)
+            st = make_rd_view(angles[v], arith, &h[v]);
+        p->h_geom = h;
+    }
+    if (st != ITOMO_OK) {
+        std::free(p->h_geom);
+        delete p;
+        return st;
+    }
+    st = check_hip(hipMalloc(&p->d_geom, bytes), "plan_create: hipMalloc");
+    if (st == ITOMO_OK)
+        st = check_hip(hipMemcpy(p->d_geom, p->h_geom, bytes, hipMemcpyHostToDevice),
+                       "plan_create: hipMemcpy");
+    if (st != ITOMO_OK) {
+        if (p->d_geom) (void)hipFree(p->d_geom);
+        std::free(p->h_geom);
+        delete p;
+        return st;
+    }
+    *out = p;
+    return ITOMO_OK;
+}
+
+int itomo_plan_destroy(itomo_plan* p)
+{
+    if (!p) return ITOMO_OK;
+    int st = ITOMO_OK;
+    if (p->d_geom) st = check_hip(hipFree(p->d_geom), "plan_destroy: hipFree");
+    if (p->scratch) {
+        int st2 = check_hip(hipFree(p->scratch), "plan_destroy: hipFree scratch");
+        if (!st) st = st2;
+    }
+    std::free(p->h_geom);
+    delete p;
+    return st;
+}
+
+int itomo_plan_trig(const itomo_plan* p, float* s_host, float* c_host)
+{
+    if (!p || !s_host || !c_host) return set_error(ITOMO_EINVAL, "plan_trig: null pointer");
+    if (p->projector == ITOMO_PROJ_RD) {
+        const RdView* g = (const RdView*)p->h_geom;
+        for (int v = 0; v < p->nang; ++v) { s_host[v] = g[v].s; c_host[v] = g[v].c; }
+    } else {
+        const DdView* g = (const DdView*)p->h_geom;
+        for (int v = 0; v < p->nang; ++v) { s_host[v] = g[v].s; c_host[v] = g[v].c; }
+    }
+    return ITOMO_OK;
+}
+
+static int check_io(const itomo_plan* p, const void* a, const void* b, int nslices,
+                    const char* who)
+{
+    if (!p) return set_error(ITOMO_EINVAL, "%s: null plan", who);
+    if (!a || !b) return set_error(ITOMO_EINVAL, "%s: null array", who);
+    if (nslices < 1 || nslices > 65535)
+        return set_error(ITOMO_EINVAL, "%s: nslices %d outside [1, 65535]", who, nslices);
+    return ITOMO_OK;
+}
+
+int itomo_forward(const itomo_plan* p, const float* image, float* sino, int nslices,
+                  int accumulate, void* stream)
+{
+    int st = check_io(p, image, sino, nslices, "forward");
+    if (st) return st;
+    const int64_t is = (int64_t)p->npix * p->npix, ss = (int64_t)p->nang * p->npix;
+    hipStream_t s = (hipStream_t)stream;
+    if (p->projector == ITOMO_PROJ_DD)
+        return dd_forward(*p, image, sino, nslices, is, ss, accumulate, s);
+    return rd_forward(*p, image, sino, nslices, is, ss, accumulate, s);
+}
+
+int itomo_back(const itomo_plan* p, const float* sino, float* image, int nslices,
+               int accumulate, void* stream)
+{
+    int st = check_io(p, sino, image, nslices, "back");
+    if (st) return st;
+    const int64_t is = (int64_t)p->npix * p->npix, ss = (int64_t)p->nang * p->npix;
+    hipStream_t s = (hipStream_t)stream;
+    if (p->projector == ITOMO_PROJ_DD)
+        return dd_back(*p, sino, image, nslices, ss, is, accumulate, s);
+    return rd_back(*p, sino, image, nslices, ss, is, accumulate, s);
+}
+
+static int one_shot(int projector, float* image, int npix, const float* angles, int nang,
+                    int oper, float* sino, void* stream)
+{
+    if (oper != 0 && oper != 1)
+        return set_error(ITOMO_EINVAL, "radon: oper must be 0 (forward) or 1 (back)");
+    itomo_plan* p = nullptr;
+    int st = itomo_plan_create(&p, projector, npix, angles, nang, ITOMO_ARITH_FAST);
+    if (st) return st;
+    st = oper == 0 ? itomo_forward(p, image, sino, 1, 1, stream)
+                   : itomo_back(p, sino, image, 1, 1, stream);
+    // the plan's device table must outlive the queued kernel
+    int st2 = check_hip(hipStreamSynchronize((hipStream_t)stream), "radon: sync");
+    int st3 = itomo_plan_destroy(p);
+    return st ? st : (st2 ? st2 : st3);
+}
+
+int itomo_radon_dd(float* image, int npix, const float* angles, int nang, int oper,
+                   float* sino, void* stream)
+{
+    return one_shot(ITOMO_PROJ_DD, image, npix, angles, nang, oper, sino, stream);
+}
+
+int itomo_radon_rd(float* image, int npix, const float* angles, int nang, int oper,
+                   float* sino, void* stream)
+{
+    return one_shot(ITOMO_PROJ_RD, image, npix, angles, nang, oper, sino, stream);
+}
+
+}  // extern "C"

@@ -1,0 +1,64 @@
+"""Write the golden projector vectors in tests/golden/radon_ref.npz.
+
+Inputs are seeded phantoms / sinograms; expected outputs come from the reference's own
+radon C (radon_dd.c, radon_rd.c) compiled from the reference sources by
+``make -C oracle ref`` into ``oracle/_ref`` -- both with the reference's flags
+(``fast``: -O3 -ffast-math) and with IEEE flags (``strict``).  The angle arrays stored
+are what the reference facade passes to the Cython wrappers (float32 radians,
+``class_projectors_radon.py:49,54``); the wrappers' fftshift is applied by
+``oracle.ref`` exactly as ``radon_distance_driven.pyx:27`` does.
+
+Run from the repository root:  python tests/golden/make_golden.py
+"""
+import os
+import sys
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, ROOT)
+
+from oracle import ref  # noqa: E402
+from iterative_tomographic_reconstruction_algorithms_amd.phantom import (  # noqa: E402
+    angles_deg, ellipses)
+
+OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "radon_ref.npz")
+
+
+def facade_angles(deg):
+    return (np.asarray(deg, dtype=np.float64) * np.pi / 180.0).astype(np.float32)
+
+
+def cases():
+    out = []
+    for proj in ("dd", "rd"):
+        for n, v in ((8, 1), (8, 4), (16, 36), (64, 36), (64, 305), (256, 304)):
+            out.append((proj, "grid", n, facade_angles(angles_deg(v))))
+        for name, theta in (("t0", 0.0), ("t45", np.pi / 4), ("t90m", np.pi / 2 - 1e-3),
+                            ("t90p", np.pi / 2 + 1e-3), ("t135", 3 * np.pi / 4),
+                            ("t180m", np.pi - 1e-3), ("t90", np.pi / 2)):
+            out.append((proj, name, 64, np.array([theta], dtype=np.float32)))
+        out.append((proj, "odd", 63, facade_angles(angles_deg(37))))
+    return out
+
+
+def main():
+    if not ref.available("dd") or not ref.available("rd"):
+        sys.exit("oracle/_ref is not built: run `make -C oracle ref` first")
+    data = {}
+    for k, (proj, name, n, ang) in enumerate(cases()):
+        img = ellipses(n, 10, seed=100 + k)
+        sino = np.random.default_rng(200 + k).random((ang.size, n)).astype(np.float32)
+        key = "%s_%s_%d_%d" % (proj, name, n, ang.size)
+        data[key + "/angles"] = ang
+        data[key + "/image"] = img
+        data[key + "/sino"] = sino
+        for flags in ("fast", "strict"):
+            data[key + "/fwd_" + flags] = ref.forward(proj, img, ang, flags)
+            data[key + "/back_" + flags] = ref.back(proj, sino, ang, flags)
+    np.savez_compressed(OUT, **data)
+    print("wrote %s (%d arrays, %.1f MB)" % (OUT, len(data), os.path.getsize(OUT) / 1e6))
+
+
+if __name__ == "__main__":
+    main()

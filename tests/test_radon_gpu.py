@@ -1,0 +1,191 @@
+"""GPU parity of the HIP projectors against the reference's own C (golden vectors) and the
+oracle, through the C ABI.
+
+Bars (stated per test):
+  * distance-driven forward and back, ray-driven forward: bitwise equal to the
+    reference build they mirror (``fast`` arithmetic vs the reference's -ffast-math
+    build, ``ieee`` vs its IEEE build);
+  * ray-driven back: the scatter applies the reference's exact per-update rounding in
+    a different order, so it is held to per-image relative L2 <= 1e-6;
+  * at the full 2048^2 x 1800 geometry, where the CPU restatement is too slow for
+    whole arrays: per-view bitwise comparison on a spread of views, exact linearity,
+    adjointness in FP64 to 1e-6.
+"""
+import os
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import radon as O
+from iterative_tomographic_reconstruction_algorithms_amd.core import Plan
+from iterative_tomographic_reconstruction_algorithms_amd import (
+    class_projectors_radon, radon_distance_driven, radon_ray_driven)
+from iterative_tomographic_reconstruction_algorithms_amd.phantom import angles_deg, ellipses
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden", "radon_ref.npz")
+Z = np.load(GOLDEN)
+KEYS = sorted({k.split("/")[0] for k in Z.files})
+
+
+def rel_l2(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+def c_angles(proj, facade_angles):
+    a = np.asarray(facade_angles, np.float32)
+    return np.fft.fftshift(a) if proj == "dd" else a
+
+
+def gpu(x):
+    return torch.from_numpy(np.ascontiguousarray(x, np.float32)).cuda()
+
+
+@pytest.mark.parametrize("arith,flags", [("fast", "fast"), ("ieee", "strict")])
+@pytest.mark.parametrize("key", KEYS)
+def test_projectors_match_reference_golden(key, arith, flags):
+    proj = key.split("_")[0]
+    ang = Z[key + "/angles"]
+    img, sino = Z[key + "/image"], Z[key + "/sino"]
+    p = Plan(proj, img.shape[0], c_angles(proj, ang), arith)
+    fwd = p.forward(gpu(img)).cpu().numpy()
+    back = p.back(gpu(sino)).cpu().numpy()
+    np.testing.assert_array_equal(fwd, Z[key + "/fwd_" + flags])
+    if proj == "dd":
+        np.testing.assert_array_equal(back, Z[key + "/back_" + flags])
+    else:
+        assert rel_l2(back, Z[key + "/back_" + flags]) <= 1e-6
+
+
+@pytest.mark.parametrize("proj", ["dd", "rd"])
+def test_batched_slices_equal_single_slices(proj):
+    n, v = 64, 45
+    ang = c_angles(proj, (angles_deg(v) * np.pi / 180).astype(np.float32))
+    p = Plan(proj, n, ang)
+    imgs = np.stack([ellipses(n, 8, seed=s) for s in range(3)])
+    fwd = p.forward(gpu(imgs)).cpu().numpy()
+    for k in range(3):
+        np.testing.assert_array_equal(fwd[k], O.forward(proj, imgs[k], np.fft.ifftshift(ang)
+                                                        if proj == "dd" else ang, O.FAST))
+    sinos = np.random.default_rng(1).random((3, v, n)).astype(np.float32)
+    back = p.back(gpu(sinos)).cpu().numpy()
+    for k in range(3):
+        single = p.back(gpu(sinos[k])).cpu().numpy()
+        if proj == "dd":
+            np.testing.assert_array_equal(back[k], single)
+        else:
+            assert rel_l2(back[k], single) <= 1e-6
+
+
+def test_accumulate_adds_into_output():
+    n, v = 32, 20
+    ang = np.fft.fftshift((angles_deg(v) * np.pi / 180).astype(np.float32))
+    p = Plan("dd", n, ang)
+    img = gpu(ellipses(n, 5, seed=2))
+    base = gpu(np.full((v, n), 3.0, np.float32))
+    out = p.forward(img, out=base.clone(), accumulate=True).cpu().numpy()
+    ref = O.dd_forward(ellipses(n, 5, seed=2), np.fft.ifftshift(ang), O.FAST)
+    # the reference C accumulates from whatever the output holds: start from 3.0
+    ref3 = np.full((v, n), 3.0, np.float32)
+    import ctypes
+    L = O.lib()
+    L.oracle_radon_dd(ellipses(n, 5, seed=2), n, ang, v, 0, ref3, O.FAST)
+    np.testing.assert_array_equal(out, ref3)
+    assert not np.array_equal(out, ref)
+
+
+def test_dd_is_deterministic():
+    n, v = 128, 90
+    ang = np.fft.fftshift((angles_deg(v) * np.pi / 180).astype(np.float32))
+    p = Plan("dd", n, ang)
+    img = gpu(ellipses(n, 10, seed=5))
+    s1, s2 = p.forward(img), p.forward(img)
+    assert torch.equal(s1, s2)
+    b1, b2 = p.back(s1), p.back(s1)
+    assert torch.equal(b1, b2)
+
+
+def test_dropin_modules_numpy_roundtrip_and_errors():
+    n, v = 32, 12
+    deg = angles_deg(v).astype(np.float32)
+    img = ellipses(n, 5, seed=8)
+    # degrees are converted in place, like the reference wrappers
+    a = deg.copy()
+    s = radon_distance_driven.forwproj(img, a)
+    np.testing.assert_allclose(a, deg * np.float32(np.pi / 180.0), rtol=0, atol=0)
+    assert isinstance(s, np.ndarray) and s.dtype == np.float32 and s.shape == (v, n)
+    np.testing.assert_array_equal(s, O.dd_forward(img, a, O.FAST))
+    b = radon_distance_driven.backproj(s, a)
+    np.testing.assert_array_equal(b, O.dd_back(s, a, O.FAST))
+    r = radon_ray_driven.forwproj(img, a)
+    np.testing.assert_array_equal(r, O.rd_forward(img, a, O.FAST))
+    with pytest.raises(ValueError):
+        radon_distance_driven.forwproj(img.astype(np.float64), a)
+    with pytest.raises(ValueError):
+        radon_distance_driven.forwproj(np.asfortranarray(img), a)
+    with pytest.raises(TypeError):
+        radon_distance_driven.forwproj(None, a)
+
+
+def test_facade_numpy_and_device_inputs():
+    n, v = 48, 30
+    deg = angles_deg(v)
+    for oper in ("dd", "rd"):
+        tp = class_projectors_radon.projectors(n, deg, oper=oper)
+        x = ellipses(n, 6, seed=12)
+        s_np = tp.A(x)
+        s_dev = tp.A(gpu(x))
+        assert isinstance(s_np, np.ndarray) and s_dev.is_cuda
+        np.testing.assert_array_equal(s_np, s_dev.cpu().numpy())
+        np.testing.assert_array_equal(s_np, O.forward(oper, x, tp.angles, O.FAST))
+        bt = tp.At(s_np)
+        assert bt.shape == (n, n)
+
+
+# ---- the full benchmark geometry ----------------------------------------------------
+
+N_FULL, V_FULL = 2048, 1800
+
+
+@pytest.fixture(scope="module")
+def full_plan():
+    ang = np.fft.fftshift((angles_deg(V_FULL) * np.pi / 180).astype(np.float32))
+    return Plan("dd", N_FULL, ang), ang
+
+
+def test_full_size_forward_views_bitwise(full_plan):
+    p, ang = full_plan
+    img = ellipses(N_FULL, 40, seed=2)
+    sino = p.forward(gpu(img)).cpu().numpy()
+    # views chosen around 0, 45, 90, 135 degrees and near the sweep poles
+    for v in (0, 1, 450, 898, 899, 900, 901, 1349, 1798, 1799):
+        ref = O.dd_forward(img, np.array([ang[v]], np.float32), O.FAST)
+        # oracle.dd_forward fftshifts a 1-element list: no-op; its row 0 is view v
+        np.testing.assert_array_equal(sino[V_FULL - 1 - v], ref[0], err_msg="view %d" % v)
+
+
+def test_full_size_back_single_views_bitwise(full_plan):
+    p, ang = full_plan
+    rng = np.random.default_rng(3)
+    for v in (3, 899, 1200, 1799):
+        sino = np.zeros((V_FULL, N_FULL), np.float32)
+        sino[V_FULL - 1 - v] = rng.random(N_FULL).astype(np.float32)
+        got = p.back(gpu(sino)).cpu().numpy()
+        ref = O.dd_back(sino[V_FULL - 1 - v][None, :], np.array([ang[v]], np.float32), O.FAST)
+        np.testing.assert_array_equal(got, ref, err_msg="view %d" % v)
+
+
+def test_full_size_linearity_and_adjointness(full_plan):
+    p, _ = full_plan
+    x = gpu(ellipses(N_FULL, 40, seed=2))
+    y = torch.rand((V_FULL, N_FULL), device="cuda", dtype=torch.float32)
+    ax = p.forward(x)
+    assert torch.equal(p.forward(2 * x), 2 * ax)  # scaling by 2 is exact
+    aty = p.back(y)
+    lhs = torch.dot(ax.double().reshape(-1), y.double().reshape(-1)).item()
+    rhs = torch.dot(x.double().reshape(-1), aty.double().reshape(-1)).item()
+    assert abs(lhs - rhs) <= 1e-6 * abs(lhs)
