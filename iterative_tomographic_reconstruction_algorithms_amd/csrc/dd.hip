@@ -43,6 +43,15 @@ __device__ __forceinline__ float key_K(const DdView& g, float u)
 
 __device__ __forceinline__ float line_coord(int i, int nh) { return (float)(i - nh) + 0.5f; }
 
+// acc += w * v as the reference build evaluates it: fused under -ffast-math (FAST),
+// a rounded product then a rounded sum as written (IEEE).
+template <int ARITH>
+__device__ __forceinline__ float accum(float w, float v, float acc)
+{
+    if (ARITH == ITOMO_ARITH_FAST) return __builtin_fmaf(w, v, acc);
+    return __fadd_rn(acc, __fmul_rn(w, v));
+}
+
 // Accumulate the pixel with boundaries [lo, hi] against all bins it overlaps, in
 // increasing bin order.  E is the dropped right end of this line.
 template <int ARITH>
@@ -71,7 +80,7 @@ __device__ __forceinline__ float pixel_gather(const DdView& g, int npix, int nh,
         if (d >= 0) {
             const float R = fminf(hi, Dhi), L = fmaxf(lo, Dlo);
             if (R > L && R != E)
-                acc = __builtin_fmaf(__fsub_rn(R, L), ones ? 1.0f : srow[d], acc);
+                acc = accum<ARITH>(__fsub_rn(R, L), ones ? 1.0f : srow[d], acc);
         }
         ++d;
         Dlo = Dhi;
@@ -195,8 +204,8 @@ dd_fwd_kernel(const DdView* __restrict__ geom, int nang, int npix,
                     if (q >= 0) {
                         const float R = fminf(Phi, Dhi), L = fmaxf(Plo, Dlo);
                         if (R > L && R != E)
-                            acc = __builtin_fmaf(__fsub_rn(R, L),
-                                                 ONES ? 1.0f : line[(int64_t)q * pix_step], acc);
+                            acc = accum<ARITH>(__fsub_rn(R, L),
+                                               ONES ? 1.0f : line[(int64_t)q * pix_step], acc);
                     }
                     ++q;
                     Plo = Phi;

@@ -133,8 +133,10 @@ __device__ float rd_walk(float acc, const float* __restrict__ img_in, float* img
 {
     const int nh = (int)(npix * 0.5);
     const float d0 = __fsub_rn(p2[0], p1[0]), d1 = __fsub_rn(p2[1], p1[1]);
+    // float sqrt via double: rounding a correctly rounded double square root to float
+    // gives the correctly rounded float square root (53 >= 2*24 + 2).
     const float incr = (ARITH == ITOMO_ARITH_FAST)
-                           ? __fsqrt_rn(__fmaf_rn(d0, d0, __fmul_rn(d1, d1)))
+                           ? (float)__dsqrt_rn((double)__fmaf_rn(d0, d0, __fmul_rn(d1, d1)))
                            : (float)__dsqrt_rn((double)__fadd_rn(__fmul_rn(d0, d0),
                                                                  __fmul_rn(d1, d1)));
     int ii = (int)rd_round<ARITH>(p1[0]);
