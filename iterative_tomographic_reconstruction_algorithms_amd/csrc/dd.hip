@@ -91,53 +91,209 @@ __device__ __forceinline__ float pixel_gather(const DdView& g, int npix, int nh,
 
 }  // namespace
 
-// Back projection: one thread per pixel (row r, column q) of slice blockIdx.z.
+// ---- back projection ----------------------------------------------------------------
+//
+// Block = 64x64 pixel tile, 256 threads, each thread a 4x4 micro-tile (rows 4*ty.., columns
+// 4*tx..).  For a row sweep a thread walks its 4 rows, 4 consecutive pixels each; for a
+// column sweep its 4 columns.  Along one line the bin walk is incremental: the first
+// pixel's bin is found once, then each next pixel continues from the previous pixel's
+// last bin.  Views are processed in chunks of kBackViews: the sinogram values the tile
+// can touch in each view (at most kBackFoot bins) are staged in LDS, with zeros for
+// bins outside [0, npix), so the walk needs no bounds checks.  The accumulation order
+// per pixel (views in list order, bins increasing) is the reference's.
+constexpr int kBackTile = kDdBackTile;
+constexpr int kBackViews = 32;
+constexpr int kBackFoot = 100;  // >= 64*(|sin|+|cos|) + 5 for every view
+
+template <int ARITH>
+__device__ __forceinline__ void staged_range(const DdView& g, int npix, int nh, int r0,
+                                             int q0, int* b_lo, int* cnt)
+{
+    int lo, hi;
+    if (g.kind == DD_ROWSUM) {
+        lo = r0;
+        hi = r0 + kBackTile - 1;
+    } else if (g.kind == DD_COLSUM) {
+        lo = q0;
+        hi = q0 + kBackTile - 1;
+    } else if (g.kind == DD_SWEEP_ROWS || g.kind == DD_SWEEP_COLS) {
+        // lines are rows (r) or columns (q); pixel boundaries along the line
+        const int l0 = (g.kind == DD_SWEEP_ROWS) ? r0 : q0;
+        const int p0 = (g.kind == DD_SWEEP_ROWS) ? q0 : r0;
+        const float ka = (float)(l0 - nh) + 0.5f, kb = ka + (float)(kBackTile - 1);
+        const float K1 = ka * g.t, K2 = kb * g.t;
+        const float kmin = (float)(p0 - nh) - fmaxf(K1, K2);
+        const float kmax = (float)(p0 + kBackTile - nh) - fminf(K1, K2);
+        float f1 = floorf(kmin * g.est), f2 = floorf(kmax * g.est);
+        f1 = fminf(fmaxf(f1, (float)(-nh - 4)), (float)(npix - nh + 4));
+        f2 = fminf(fmaxf(f2, (float)(-nh - 4)), (float)(npix - nh + 4));
+        lo = (int)f1 + nh - 2;
+        hi = (int)f2 + nh + 2;
+    } else {
+        lo = 0;
+        hi = -1;
+    }
+    lo = max(lo, -1);
+    hi = min(hi, npix);
+    *b_lo = lo;
+    *cnt = max(0, min(hi - lo + 1, kBackFoot));
+}
+
+// One line of 4 pixels with along-line indices p0..p0+3.  S points at LDS such that
+// S[d] is bin d of this view (zero outside [0, npix)).
+template <int ARITH>
+__device__ __forceinline__ void walk4(const DdView& g, int npix, int nh, float K, float E,
+                                      int p0, const float* S, float& a0, float& a1,
+                                      float& a2, float& a3)
+{
+    float x = (float)(p0 - nh);
+    float lo = __fsub_rn(x, K);
+    float fe = floorf(__fmul_rn(lo, g.est));
+    fe = fminf(fmaxf(fe, (float)(-nh - 1)), (float)(npix - nh));
+    int d = (int)fe + nh;  // in [-1, npix]
+    float Dlo = key_D<ARITH>(g, d - nh);
+    float Dhi = key_D<ARITH>(g, d + 1 - nh);
+    while (d > -1 && lo < Dlo) {
+        --d;
+        Dhi = Dlo;
+        Dlo = key_D<ARITH>(g, d - nh);
+    }
+    while (d < npix && lo >= Dhi) {
+        ++d;
+        Dlo = Dhi;
+        Dhi = key_D<ARITH>(g, d + 1 - nh);
+    }
+    // invariant from here on: Dlo <= lo <= Dhi
+#define ITOMO_PIX(acc)                                                               \
+    {                                                                                \
+        x = __fadd_rn(x, 1.0f);                                                      \
+        const float hi = __fsub_rn(x, K);                                            \
+        {                                                                            \
+            const float R = fminf(hi, Dhi), L = fmaxf(lo, Dlo);                      \
+            if (R > L && R != E) acc = accum<ARITH>(__fsub_rn(R, L), S[d], acc);     \
+        }                                                                            \
+        while (d < npix && Dhi < hi) {                                               \
+            ++d;                                                                     \
+            Dlo = Dhi;                                                               \
+            Dhi = key_D<ARITH>(g, d + 1 - nh);                                       \
+            const float R = fminf(hi, Dhi);                                          \
+            if (R > Dlo && R != E) acc = accum<ARITH>(__fsub_rn(R, Dlo), S[d], acc); \
+        }                                                                            \
+        lo = hi;                                                                     \
+    }
+    ITOMO_PIX(a0)
+    ITOMO_PIX(a1)
+    ITOMO_PIX(a2)
+    ITOMO_PIX(a3)
+#undef ITOMO_PIX
+}
+
 template <int ARITH, int EPI, bool ONES>
 __global__ void __launch_bounds__(256)
 dd_back_kernel(const DdView* __restrict__ geom, int nang, int npix,
                const float* __restrict__ sino, float* __restrict__ out,
                int64_t sino_stride, int64_t img_stride, int accumulate, EpiArgs e)
 {
-    const int q = blockIdx.x * blockDim.x + threadIdx.x;  // column
-    const int r = blockIdx.y * blockDim.y + threadIdx.y;  // row
-    const bool active = q < npix && r < npix;
+    __shared__ float sS[kBackViews][kBackFoot];
+    __shared__ int sLo[kBackViews];
+    __shared__ int sCnt[kBackViews];
+
+    const int tid = threadIdx.x;
+    const int tx = tid & 15, ty = tid >> 4;
+    const int r0 = blockIdx.y * kBackTile, q0 = blockIdx.x * kBackTile;
+    const int rr = r0 + 4 * ty, cc = q0 + 4 * tx;  // micro-tile origin
     const int z = blockIdx.z;
     sino += (int64_t)z * sino_stride;
     out += (int64_t)z * img_stride;
     const int nh = (int)(npix * 0.5);
-    const int64_t o = (int64_t)r * npix + q;
-    double p1 = 0.0, p2 = 0.0;
+    const float xN = (float)(npix - nh);
 
-    if (active) {
-        float acc = (EPI == EPI_STORE && accumulate) ? out[o] : 0.0f;
-        const float ur = line_coord(r, nh), uq = line_coord(q, nh);
-        const float xr0 = (float)(r - nh), xr1 = (float)(r + 1 - nh);
-        const float xq0 = (float)(q - nh), xq1 = (float)(q + 1 - nh);
-        const float xN = (float)(npix - nh);
-        for (int v = 0; v < nang; ++v) {
-            const DdView g = geom[v];
-            const float* srow = sino + (int64_t)g.row * npix;
+    float acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int r = rr + i, q = cc + j;
+            acc[i][j] = (EPI == EPI_STORE && accumulate && r < npix && q < npix)
+                            ? out[(int64_t)r * npix + q]
+                            : 0.0f;
+        }
+
+    for (int v0 = 0; v0 < nang; v0 += kBackViews) {
+        const int nv = min(kBackViews, nang - v0);
+        __syncthreads();  // previous chunk fully consumed
+        if (tid < nv) {
+            int lo, cnt;
+            staged_range<ARITH>(geom[v0 + tid], npix, nh, r0, q0, &lo, &cnt);
+            sLo[tid] = lo;
+            sCnt[tid] = cnt;
+        }
+        __syncthreads();
+        for (int k = tid; k < nv * kBackFoot; k += 256) {
+            const int v = k / kBackFoot, j = k - v * kBackFoot;
+            const int d = sLo[v] + j;
+            float val = 0.0f;
+            if (j < sCnt[v] && d >= 0 && d < npix)
+                val = ONES ? 1.0f : sino[(int64_t)geom[v0 + v].row * npix + d];
+            sS[v][j] = val;
+        }
+        __syncthreads();
+        for (int v = 0; v < nv; ++v) {
+            const DdView g = geom[v0 + v];
+            const float* S = &sS[v][0] - sLo[v];
             if (g.kind == DD_SWEEP_ROWS) {
-                const float K = key_K<ARITH>(g, ur);
-                const float E = fminf(__fsub_rn(xN, K), key_D<ARITH>(g, npix - nh));
-                acc = pixel_gather<ARITH>(g, npix, nh, __fsub_rn(xq0, K), __fsub_rn(xq1, K),
-                                          E, srow, ONES, acc);
+                const float DN = key_D<ARITH>(g, npix - nh);
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const float K = key_K<ARITH>(g, line_coord(rr + i, nh));
+                    const float E = fminf(__fsub_rn(xN, K), DN);
+                    walk4<ARITH>(g, npix, nh, K, E, cc, S, acc[i][0], acc[i][1], acc[i][2],
+                                 acc[i][3]);
+                }
             } else if (g.kind == DD_SWEEP_COLS) {
-                const float K = key_K<ARITH>(g, uq);
-                const float E = fminf(__fsub_rn(xN, K), key_D<ARITH>(g, npix - nh));
-                acc = pixel_gather<ARITH>(g, npix, nh, __fsub_rn(xr0, K), __fsub_rn(xr1, K),
-                                          E, srow, ONES, acc);
+                const float DN = key_D<ARITH>(g, npix - nh);
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const float K = key_K<ARITH>(g, line_coord(cc + j, nh));
+                    const float E = fminf(__fsub_rn(xN, K), DN);
+                    walk4<ARITH>(g, npix, nh, K, E, rr, S, acc[0][j], acc[1][j], acc[2][j],
+                                 acc[3][j]);
+                }
             } else if (g.kind == DD_ROWSUM) {
-                acc = __fadd_rn(acc, ONES ? 1.0f : srow[r]);
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const float sv = (rr + i < npix) ? S[rr + i] : 0.0f;
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) acc[i][j] = __fadd_rn(acc[i][j], sv);
+                }
             } else if (g.kind == DD_COLSUM) {
-                acc = __fadd_rn(acc, ONES ? 1.0f : srow[q]);
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const float sv = (cc + j < npix) ? S[cc + j] : 0.0f;
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) acc[i][j] = __fadd_rn(acc[i][j], sv);
+                }
             }
         }
-        if (EPI == EPI_SIRT_BACK || EPI == EPI_MLEM_BACK)
-            out[o] = epi_back<EPI>(acc, out[o], o, e, &p1, &p2);
-        else
-            out[o] = epi_back<EPI>(acc, 0.0f, o, e, &p1, &p2);
     }
+
+    double p1 = 0.0, p2 = 0.0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int r = rr + i, q = cc + j;
+            if (r < npix && q < npix) {
+                const int64_t o = (int64_t)r * npix + q;
+                double q1 = 0.0, q2 = 0.0;
+                if (EPI == EPI_SIRT_BACK || EPI == EPI_MLEM_BACK)
+                    out[o] = epi_back<EPI>(acc[i][j], out[o], o, e, &q1, &q2);
+                else
+                    out[o] = epi_back<EPI>(acc[i][j], 0.0f, o, e, &q1, &q2);
+                p1 += q1;
+                p2 += q2;
+            }
+        }
     if (epi_has_partials(EPI) && e.partials) {
         const int blk = blockIdx.y * gridDim.x + blockIdx.x;
         block_partials3(0.0, p1, p2,
@@ -146,17 +302,22 @@ dd_back_kernel(const DdView* __restrict__ geom, int nang, int npix,
 }
 
 // Forward projection: one thread per (bin, view blockIdx.y, slice blockIdx.z).
+// Lines are image rows for row sweeps and image columns for column sweeps; column
+// sweeps (and row sums) read the transposed image so that the lanes of a wave, which
+// hold consecutive bins, read consecutive addresses.
 template <int ARITH, int EPI, bool ONES>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(kDdFwdBlock)
 dd_fwd_kernel(const DdView* __restrict__ geom, int nang, int npix,
-              const float* __restrict__ image, float* __restrict__ out,
-              int64_t img_stride, int64_t sino_stride, int accumulate, EpiArgs e)
+              const float* __restrict__ image, const float* __restrict__ imageT,
+              float* __restrict__ out, int64_t img_stride, int64_t sino_stride,
+              int accumulate, EpiArgs e)
 {
     const int d = blockIdx.x * blockDim.x + threadIdx.x;
     const bool active = d < npix;
     const int z = blockIdx.z;
     const DdView g = geom[blockIdx.y];
     image += (int64_t)z * img_stride;
+    imageT += (int64_t)z * img_stride;
     const int64_t soff = (int64_t)z * sino_stride;
     const int64_t o = (int64_t)g.row * npix + d;
     const int nh = (int)(npix * 0.5);
@@ -165,51 +326,70 @@ dd_fwd_kernel(const DdView* __restrict__ geom, int nang, int npix,
     if (active) {
         float acc = (EPI == EPI_STORE && accumulate) ? out[soff + o] : 0.0f;
         if (g.kind == DD_ROWSUM) {
-            const float* p = image + (int64_t)d * npix;
-            for (int j = 0; j < npix; ++j) acc = __fadd_rn(acc, ONES ? 1.0f : p[j]);
+            // sum_j img[d][j], in j order, read as column d of the transpose
+            const float* p = imageT + d;
+            for (int j = 0; j < npix; ++j)
+                acc = __fadd_rn(acc, ONES ? 1.0f : p[(int64_t)j * npix]);
         } else if (g.kind == DD_COLSUM) {
             const float* p = image + d;
             for (int j = 0; j < npix; ++j)
                 acc = __fadd_rn(acc, ONES ? 1.0f : p[(int64_t)j * npix]);
         } else if (g.kind == DD_SWEEP_ROWS || g.kind == DD_SWEEP_COLS) {
-            // line i: image row i (rows) or column i (columns); pixel q along it
-            const int64_t line_step = (g.kind == DD_SWEEP_ROWS) ? npix : 1;
-            const int64_t pix_step = (g.kind == DD_SWEEP_ROWS) ? 1 : npix;
+            const float* src = (g.kind == DD_SWEEP_ROWS) ? image : imageT;
             const float Dlo = key_D<ARITH>(g, d - nh);
             const float Dhi = key_D<ARITH>(g, d + 1 - nh);
             const float DN = key_D<ARITH>(g, npix - nh);
             const float x0 = (float)(-nh), xN = (float)(npix - nh);
-            for (int i = 0; i < npix; ++i) {
+            // lines whose pixel span [x0 - K, xN - K] can meet [Dlo, Dhi]:
+            // x0 - Dhi < K_i < xN - Dlo with K_i ~ (i - nh + 0.5) * t; two lines of margin,
+            // the exact test below decides.
+            double ua = ((double)x0 - (double)Dhi) / (double)g.t;
+            double ub = ((double)xN - (double)Dlo) / (double)g.t;
+            if (ua > ub) { const double tmp = ua; ua = ub; ub = tmp; }
+            ua = fmin(fmax(ua + nh - 0.5, -4.0), (double)npix + 4.0);
+            ub = fmin(fmax(ub + nh - 0.5, -4.0), (double)npix + 4.0);
+            const int i_lo = max(0, (int)floor(ua) - 2);
+            const int i_hi = min(npix - 1, (int)ceil(ub) + 2);
+            const int nc = g.ncand;
+            for (int i = i_lo; i <= i_hi; ++i) {
                 const float K = key_K<ARITH>(g, line_coord(i, nh));
                 const float P0 = __fsub_rn(x0, K), PN = __fsub_rn(xN, K);
                 if (!(PN > Dlo && P0 < Dhi)) continue;
                 const float E = fminf(PN, DN);
+                // the pixel whose span holds Dlo: estimate, then one exact correction
                 float fe = floorf(__fadd_rn(Dlo, K));
                 fe = fminf(fmaxf(fe, (float)(-nh - 1)), (float)(npix - nh));
                 int q = (int)fe + nh;  // in [-1, npix]
-                float Plo = __fsub_rn((float)(q - nh), K);
-                float Phi = __fsub_rn((float)(q + 1 - nh), K);
-                while (q > -1 && Dlo < Plo) {
-                    --q;
-                    Phi = Plo;
-                    Plo = __fsub_rn((float)(q - nh), K);
+                {
+                    const float Pe = __fsub_rn((float)(q - nh), K);
+                    const float Pe1 = __fsub_rn((float)(q + 1 - nh), K);
+                    q += (Dlo >= Pe1) ? 1 : ((Dlo < Pe) ? -1 : 0);
+                    // a bin starting before pixel 0 first meets pixel 0
+                    q = min(max(q, 0), npix - 1);
                 }
-                while (q < npix && Dlo >= Phi) {
-                    ++q;
-                    Plo = Phi;
-                    Phi = __fsub_rn((float)(q + 1 - nh), K);
-                }
-                const float* line = image + (int64_t)i * line_step;
-                while (q < npix && Plo < Dhi) {
-                    if (q >= 0) {
-                        const float R = fminf(Phi, Dhi), L = fmaxf(Plo, Dlo);
-                        if (R > L && R != E)
-                            acc = accum<ARITH>(__fsub_rn(R, L),
-                                               ONES ? 1.0f : line[(int64_t)q * pix_step], acc);
+                // candidates q .. q+nc-1, four loads in flight per round
+                const float* line = src + (int64_t)i * npix;
+                float x = (float)(q - nh);
+                float Plo = __fsub_rn(x, K);
+                for (int k0 = 0; k0 < nc; k0 += 4) {
+                    float val[4];
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) {
+                        const int qq = min(max(q + k0 + k, 0), npix - 1);
+                        val[k] = ONES ? 1.0f : line[qq];
                     }
-                    ++q;
-                    Plo = Phi;
-                    Phi = __fsub_rn((float)(q + 1 - nh), K);
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) {
+                        x = __fadd_rn(x, 1.0f);
+                        const float Phi = __fsub_rn(x, K);
+                        const int qk = q + k0 + k;
+                        const float R = (Phi < Dhi) ? Phi : Dhi;
+                        const float L = (Plo > Dlo) ? Plo : Dlo;
+                        const bool ok = (k0 + k < nc) && qk >= 0 && qk < npix && R > L && R != E;
+                        const float nacc = accum<ARITH>(__fsub_rn(R, L), val[k], acc);
+                        acc = ok ? nacc : acc;
+                        Plo = Phi;
+                    }
                 }
             }
         }
@@ -222,45 +402,66 @@ dd_fwd_kernel(const DdView* __restrict__ geom, int nang, int npix,
     }
 }
 
+// out[c][r] = in[r][c] for each npix x npix slice (32x32 tiles through LDS).
+__global__ void __launch_bounds__(256)
+transpose_kernel(const float* __restrict__ in, float* __restrict__ out, int npix,
+                 int64_t stride)
+{
+    __shared__ float tile[32][33];
+    in += (int64_t)blockIdx.z * stride;
+    out += (int64_t)blockIdx.z * stride;
+    const int c0 = blockIdx.x * 32, r0 = blockIdx.y * 32;
+    for (int k = threadIdx.y; k < 32; k += 8) {
+        const int r = r0 + k, c = c0 + threadIdx.x;
+        if (r < npix && c < npix) tile[k][threadIdx.x] = in[(int64_t)r * npix + c];
+    }
+    __syncthreads();
+    for (int k = threadIdx.y; k < 32; k += 8) {
+        const int r = c0 + k, c = r0 + threadIdx.x;  // output row = input column
+        if (r < npix && c < npix) out[(int64_t)r * npix + c] = tile[threadIdx.x][k];
+    }
+}
+
 namespace {
 
 template <int ARITH, int EPI, bool ONES>
-void launch_fwd(const Plan& p, const float* image, float* out, int nslices, int accumulate,
-                const EpiArgs& e, hipStream_t st)
+void launch_fwd(const Plan& p, const float* image, const float* imageT, float* out,
+                int nslices, int accumulate, const EpiArgs& e, hipStream_t st)
 {
-    dim3 block(256), grid((p.npix + 255) / 256, p.nang, nslices);
+    dim3 block(kDdFwdBlock), grid((p.npix + kDdFwdBlock - 1) / kDdFwdBlock, p.nang, nslices);
     dd_fwd_kernel<ARITH, EPI, ONES><<<grid, block, 0, st>>>(
-        (const DdView*)p.d_geom, p.nang, p.npix, image, out, (int64_t)p.npix * p.npix,
-        (int64_t)p.nang * p.npix, accumulate, e);
+        (const DdView*)p.d_geom, p.nang, p.npix, image, imageT, out,
+        (int64_t)p.npix * p.npix, (int64_t)p.nang * p.npix, accumulate, e);
 }
 
 template <int ARITH, int EPI, bool ONES>
 void launch_back(const Plan& p, const float* sino, float* out, int nslices, int accumulate,
                  const EpiArgs& e, hipStream_t st)
 {
-    dim3 block(64, 4), grid((p.npix + 63) / 64, (p.npix + 3) / 4, nslices);
+    dim3 block(256),
+        grid((p.npix + kBackTile - 1) / kBackTile, (p.npix + kBackTile - 1) / kBackTile, nslices);
     dd_back_kernel<ARITH, EPI, ONES><<<grid, block, 0, st>>>(
         (const DdView*)p.d_geom, p.nang, p.npix, sino, out, (int64_t)p.nang * p.npix,
         (int64_t)p.npix * p.npix, accumulate, e);
 }
 
 template <int ARITH>
-int dispatch_fwd(const Plan& p, int epi, int ones, const float* image, float* out,
-                 int nslices, int accumulate, const EpiArgs& e, hipStream_t st)
+int dispatch_fwd(const Plan& p, int epi, int ones, const float* image, const float* imageT,
+                 float* out, int nslices, int accumulate, const EpiArgs& e, hipStream_t st)
 {
     switch (epi) {
     case EPI_STORE:
-        if (ones) launch_fwd<ARITH, EPI_STORE, true>(p, image, out, nslices, accumulate, e, st);
-        else launch_fwd<ARITH, EPI_STORE, false>(p, image, out, nslices, accumulate, e, st);
+        if (ones) launch_fwd<ARITH, EPI_STORE, true>(p, image, imageT, out, nslices, accumulate, e, st);
+        else launch_fwd<ARITH, EPI_STORE, false>(p, image, imageT, out, nslices, accumulate, e, st);
         return ITOMO_OK;
     case EPI_RECIP:
-        launch_fwd<ARITH, EPI_RECIP, true>(p, image, out, nslices, 0, e, st);
+        launch_fwd<ARITH, EPI_RECIP, true>(p, image, imageT, out, nslices, 0, e, st);
         return ITOMO_OK;
     case EPI_SIRT_FWD:
-        launch_fwd<ARITH, EPI_SIRT_FWD, false>(p, image, out, nslices, 0, e, st);
+        launch_fwd<ARITH, EPI_SIRT_FWD, false>(p, image, imageT, out, nslices, 0, e, st);
         return ITOMO_OK;
     case EPI_MLEM_FWD:
-        launch_fwd<ARITH, EPI_MLEM_FWD, false>(p, image, out, nslices, 0, e, st);
+        launch_fwd<ARITH, EPI_MLEM_FWD, false>(p, image, imageT, out, nslices, 0, e, st);
         return ITOMO_OK;
     }
     return set_error(ITOMO_EINVAL, "dd forward: unsupported epilogue %d", epi);
@@ -294,12 +495,25 @@ int dd_forward_epi(const Plan& p, int epi, int ones, const float* image, float* 
                    int nslices, int accumulate, const EpiArgs& e, hipStream_t st,
                    int* blocks_per_slice)
 {
-    if (blocks_per_slice) *blocks_per_slice = ((p.npix + 255) / 256) * p.nang;
+    const int bps = (int)fwd_blocks_per_slice(p);
+    if (blocks_per_slice) *blocks_per_slice = bps;
     EpiArgs ee = e;
-    ee.blocks_per_slice = ((p.npix + 255) / 256) * p.nang;
+    ee.blocks_per_slice = bps;
+    const float* imageT = image;
+    if (!ones) {
+        Scratch s;
+        int rc = get_scratch(p, nslices, &s, st);
+        if (rc) return rc;
+        const int64_t n2 = (int64_t)p.npix * p.npix;
+        dim3 tb(32, 8), tg((p.npix + 31) / 32, (p.npix + 31) / 32, nslices);
+        transpose_kernel<<<tg, tb, 0, st>>>(image, s.xT, p.npix, n2);
+        rc = check_hip(hipGetLastError(), "transpose launch");
+        if (rc) return rc;
+        imageT = s.xT;
+    }
     int rc = p.arith == ITOMO_ARITH_FAST
-                 ? dispatch_fwd<ITOMO_ARITH_FAST>(p, epi, ones, image, out, nslices, accumulate, ee, st)
-                 : dispatch_fwd<ITOMO_ARITH_IEEE>(p, epi, ones, image, out, nslices, accumulate, ee, st);
+                 ? dispatch_fwd<ITOMO_ARITH_FAST>(p, epi, ones, image, imageT, out, nslices, accumulate, ee, st)
+                 : dispatch_fwd<ITOMO_ARITH_IEEE>(p, epi, ones, image, imageT, out, nslices, accumulate, ee, st);
     if (rc) return rc;
     return check_hip(hipGetLastError(), "dd forward launch");
 }
@@ -307,7 +521,7 @@ int dd_forward_epi(const Plan& p, int epi, int ones, const float* image, float* 
 int dd_back_epi(const Plan& p, int epi, int ones, const float* sino, float* out, int nslices,
                 int accumulate, const EpiArgs& e, hipStream_t st, int* blocks_per_slice)
 {
-    const int bps = ((p.npix + 63) / 64) * ((p.npix + 3) / 4);
+    const int bps = (int)back_blocks_per_slice(p);
     if (blocks_per_slice) *blocks_per_slice = bps;
     EpiArgs ee = e;
     ee.blocks_per_slice = bps;

@@ -60,7 +60,6 @@ int finalize_partials(const double* partials, int bps, int nslices, int from, in
 }
 
 // Elementwise back epilogue over nslices images of n pixels: x <- epi(acc, x).
-constexpr int kEpiChunk = 256 * 8;
 
 template <int EPI>
 __global__ void __launch_bounds__(256)
@@ -160,37 +159,11 @@ int projector_back_epi(const Plan& p, int epi, int ones, const float* sino, floa
     return elementwise_back_epilogue(epi, tmp, x, e, n, nslices, st);
 }
 
-size_t fwd_bps(const Plan& p)
-{
-    return p.projector == ITOMO_PROJ_DD ? (size_t)((p.npix + 255) / 256) * p.nang
-                                        : (size_t)((p.npix + 63) / 64) * p.nang;
-}
+using IterScratch = Scratch;
 
-size_t back_bps(const Plan& p)
+int iter_scratch(const Plan& p, int nslices, bool, IterScratch* s, hipStream_t st)
 {
-    const int64_t n = (int64_t)p.npix * p.npix;
-    return p.projector == ITOMO_PROJ_DD ? (size_t)((p.npix + 63) / 64) * ((p.npix + 3) / 4)
-                                        : (size_t)((n + kEpiChunk - 1) / kEpiChunk);
-}
-
-struct IterScratch {
-    double* pf;
-    double* pb;
-    float* img;
-};
-
-int iter_scratch(const Plan& p, int nslices, bool with_stats, IterScratch* s, hipStream_t st)
-{
-    size_t nf = with_stats ? fwd_bps(p) * 3 * (size_t)nslices : 0;
-    size_t nb = with_stats ? back_bps(p) * 3 * (size_t)nslices : 0;
-    size_t ni = p.projector == ITOMO_PROJ_RD ? (size_t)p.npix * p.npix * (size_t)nslices : 0;
-    void* base = nullptr;
-    int rc = plan_scratch(p, (nf + nb) * sizeof(double) + ni * sizeof(float) + 256, &base, st);
-    if (rc) return rc;
-    s->pf = (double*)base;
-    s->pb = s->pf + nf;
-    s->img = (float*)(s->pb + nb);
-    return ITOMO_OK;
+    return get_scratch(p, nslices, s, st);
 }
 
 int check_iter(const itomo_plan* p, int nslices, const char* who)

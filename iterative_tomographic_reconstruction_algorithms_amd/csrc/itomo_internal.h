@@ -38,7 +38,8 @@ struct DdView {
     float a, b;  // IEEE form: (cos, sin) for rows, (sin, cos) for columns
     float est;   // b, used only to estimate a bin index (result is corrected exactly)
     float s, c;  // sin, cos of the view as the reference computes them
-    float pad[2];
+    int ncand;   // pixels that can overlap one bin on one line: floor(rinv + 0.25) + 2
+    float pad;
 };
 
 struct RdView {
@@ -64,6 +65,38 @@ struct Plan {
 
 // Returns plan scratch of at least `bytes` bytes (allocating on first use / growth).
 int plan_scratch(const Plan& p, size_t bytes, void** ptr, hipStream_t st);
+
+// Launch geometry shared by the launchers and the scratch layout.
+constexpr int kDdFwdBlock = 256;            // bins per forward block
+constexpr int kDdBackTile = 64;             // back block = 64x64 pixel tile
+constexpr int kRdBlock = 64;
+constexpr int kEpiChunk = 256 * 8;          // pixels per elementwise-epilogue block
+
+inline size_t fwd_blocks_per_slice(const Plan& p)
+{
+    return p.projector == ITOMO_PROJ_DD
+               ? (size_t)((p.npix + kDdFwdBlock - 1) / kDdFwdBlock) * p.nang
+               : (size_t)((p.npix + kRdBlock - 1) / kRdBlock) * p.nang;
+}
+
+inline size_t back_blocks_per_slice(const Plan& p)
+{
+    const size_t n2 = (size_t)p.npix * p.npix;
+    return p.projector == ITOMO_PROJ_DD
+               ? (size_t)((p.npix + kDdBackTile - 1) / kDdBackTile) *
+                     (size_t)((p.npix + kDdBackTile - 1) / kDdBackTile)
+               : (n2 + kEpiChunk - 1) / kEpiChunk;
+}
+
+// Per-call device scratch carved from the plan's buffer.  The layout depends only on
+// (plan, nslices), so every call of one iteration sees the same addresses.
+struct Scratch {
+    float* xT;     // nslices transposed images (distance-driven forward)
+    double* pf;    // forward-kernel FP64 partials [slice][block][3]
+    double* pb;    // back-kernel FP64 partials    [slice][block][3]
+    float* img;    // nslices images (ray-driven scatter target)
+};
+int get_scratch(const Plan& p, int nslices, Scratch* s, hipStream_t st);
 
 // error reporting
 int set_error(int status, const char* fmt, ...);

@@ -67,6 +67,7 @@ static int make_dd_view(float theta, int arith, int row, DdView* g)
                              (double)theta);
         g->kind = DD_SWEEP_ROWS;
         g->rinv = 1.0f / s;
+        g->ncand = (int)std::floor((double)g->rinv + 0.25) + 2;
         g->t = c * g->rinv;
         g->a = c;
         g->b = s;
@@ -79,6 +80,7 @@ static int make_dd_view(float theta, int arith, int row, DdView* g)
                              (double)theta);
         g->kind = DD_SWEEP_COLS;
         g->rinv = 1.0f / c;
+        g->ncand = (int)std::floor((double)g->rinv + 0.25) + 2;
         g->t = s * g->rinv;
         g->a = s;
         g->b = c;
@@ -128,6 +130,29 @@ int plan_scratch(const Plan& p, size_t bytes, void** ptr, hipStream_t st)
         p.scratch_bytes = want;
     }
     *ptr = p.scratch;
+    return ITOMO_OK;
+}
+
+int get_scratch(const Plan& p, int nslices, Scratch* s, hipStream_t st)
+{
+    const size_t ns = (size_t)nslices, n2 = (size_t)p.npix * p.npix;
+    const size_t nxt = n2 * ns;
+    const size_t nf = fwd_blocks_per_slice(p) * 3 * ns;
+    const size_t nb = back_blocks_per_slice(p) * 3 * ns;
+    const size_t ni = p.projector == ITOMO_PROJ_RD ? n2 * ns : 0;
+    auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    const size_t o_pf = up(nxt * sizeof(float));
+    const size_t o_pb = o_pf + up(nf * sizeof(double));
+    const size_t o_img = o_pb + up(nb * sizeof(double));
+    const size_t total = o_img + up(ni * sizeof(float));
+    void* base = nullptr;
+    int rc = plan_scratch(p, total, &base, st);
+    if (rc) return rc;
+    char* b = (char*)base;
+    s->xT = (float*)b;
+    s->pf = (double*)(b + o_pf);
+    s->pb = (double*)(b + o_pb);
+    s->img = (float*)(b + o_img);
     return ITOMO_OK;
 }
 
