@@ -301,6 +301,38 @@ dd_back_kernel(const DdView* __restrict__ geom, int nang, int npix,
     }
 }
 
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t image_rsrc(const float* base, int npix)
+{
+    // byte size; npix <= 32766 keeps it below 2^32
+    const unsigned bytes = (unsigned)npix * (unsigned)npix * 4u;
+    return __builtin_amdgcn_make_buffer_rsrc((void*)base, (short)0, (int)bytes, 0x00020000);
+}
+
+// image[idx] of one slice; indices outside [0, npix*npix) read 0.
+__device__ __forceinline__ float load_px(__amdgpu_buffer_rsrc_t rs, int idx)
+{
+    // the builtin returns the 32 bits as an integer: reinterpret, do not convert
+    return __int_as_float(
+        (int)__builtin_amdgcn_raw_buffer_load_b32(rs, (int)((unsigned)idx * 4u), 0, 0));
+}
+
+// One candidate pixel of a bin [Dlo, Dhi] on a line with offset K: the pixel spans
+// [Plo, x+1-K].  Only the first candidate can start left of Dlo.  Advances x and Plo.
+template <int ARITH>
+__device__ __forceinline__ float cand_step(bool first, bool in_range, float& x, float& Plo,
+                                           float K, float Dlo, float Dhi, float E, float val,
+                                           float acc)
+{
+    x = __fadd_rn(x, 1.0f);
+    const float Phi = __fsub_rn(x, K);
+    const float R = (Phi < Dhi) ? Phi : Dhi;
+    const float L = first ? ((Plo > Dlo) ? Plo : Dlo) : Plo;
+    const bool ok = in_range && R > L && R != E;
+    const float nacc = accum<ARITH>(__fsub_rn(R, L), val, acc);
+    Plo = Phi;
+    return ok ? nacc : acc;
+}
+
 // Forward projection: one thread per (bin, view blockIdx.y, slice blockIdx.z).
 // Lines are image rows for row sweeps and image columns for column sweeps; column
 // sweeps (and row sums) read the transposed image so that the lanes of a wave, which
@@ -335,7 +367,8 @@ dd_fwd_kernel(const DdView* __restrict__ geom, int nang, int npix,
             for (int j = 0; j < npix; ++j)
                 acc = __fadd_rn(acc, ONES ? 1.0f : p[(int64_t)j * npix]);
         } else if (g.kind == DD_SWEEP_ROWS || g.kind == DD_SWEEP_COLS) {
-            const float* src = (g.kind == DD_SWEEP_ROWS) ? image : imageT;
+            const __amdgpu_buffer_rsrc_t rs =
+                image_rsrc((g.kind == DD_SWEEP_ROWS) ? image : imageT, npix);
             const float Dlo = key_D<ARITH>(g, d - nh);
             const float Dhi = key_D<ARITH>(g, d + 1 - nh);
             const float DN = key_D<ARITH>(g, npix - nh);
@@ -367,28 +400,28 @@ dd_fwd_kernel(const DdView* __restrict__ geom, int nang, int npix,
                     // a bin starting before pixel 0 first meets pixel 0
                     q = min(max(q, 0), npix - 1);
                 }
-                // candidates q .. q+nc-1, four loads in flight per round
-                const float* line = src + (int64_t)i * npix;
+                // candidates q .. q+nc-1; loads go through a buffer descriptor whose
+                // range check returns 0 outside the image, so no index clamping
+                const int lbase = i * npix;
                 float x = (float)(q - nh);
                 float Plo = __fsub_rn(x, K);
-                for (int k0 = 0; k0 < nc; k0 += 4) {
-                    float val[4];
+                if (nc <= 3) {
+                    float v[3];
 #pragma unroll
-                    for (int k = 0; k < 4; ++k) {
-                        const int qq = min(max(q + k0 + k, 0), npix - 1);
-                        val[k] = ONES ? 1.0f : line[qq];
-                    }
+                    for (int k = 0; k < 3; ++k) v[k] = ONES ? 1.0f : load_px(rs, lbase + q + k);
 #pragma unroll
-                    for (int k = 0; k < 4; ++k) {
-                        x = __fadd_rn(x, 1.0f);
-                        const float Phi = __fsub_rn(x, K);
-                        const int qk = q + k0 + k;
-                        const float R = (Phi < Dhi) ? Phi : Dhi;
-                        const float L = (Plo > Dlo) ? Plo : Dlo;
-                        const bool ok = (k0 + k < nc) && qk >= 0 && qk < npix && R > L && R != E;
-                        const float nacc = accum<ARITH>(__fsub_rn(R, L), val[k], acc);
-                        acc = ok ? nacc : acc;
-                        Plo = Phi;
+                    for (int k = 0; k < 3; ++k)
+                        acc = cand_step<ARITH>(k == 0, q + k < npix, x, Plo, K, Dlo, Dhi, E, v[k], acc);
+                } else {
+                    for (int k0 = 0; k0 < nc; k0 += 4) {
+                        float v[4];
+#pragma unroll
+                        for (int k = 0; k < 4; ++k)
+                            v[k] = ONES ? 1.0f : load_px(rs, lbase + q + k0 + k);
+#pragma unroll
+                        for (int k = 0; k < 4; ++k)
+                            acc = cand_step<ARITH>(k0 + k == 0, (k0 + k < nc) && (q + k0 + k < npix),
+                                                   x, Plo, K, Dlo, Dhi, E, v[k], acc);
                     }
                 }
             }
