@@ -130,6 +130,56 @@ int itomo_mlem_iterate(const itomo_plan *plan, float *x, const float *b,
                        void *stream);
 
 /* ---------------------------------------------------------------------------------
+ * SPS without penalty (algorithms/sir.py:348-540, beta = 0):
+ *   setup: w = exp(-b / max(b)) over all slices, den = A^T(w * A(1)) per slice
+ *   step:  work = (A x - b) * w;  x <- x - A^T(work) / (den + 2^-52), the quotient in FP64
+ * w: nslices x nang x npix, den: nslices x npix x npix, work: nslices x nang x npix,
+ * bmax_work: DEVICE float[1024].  stats as for SIRT with [0] = sum work^2.
+ * ------------------------------------------------------------------------------- */
+int itomo_sps_weights(const itomo_plan *plan, const float *b, int nslices, float *w,
+                      float *den, float *work, float *bmax_work, void *stream);
+int itomo_sps_iterate(const itomo_plan *plan, float *x, const float *b, const float *w,
+                      const float *den, float *work, int nslices, double *stats,
+                      void *stream);
+
+/* ---------------------------------------------------------------------------------
+ * Conjugate gradients on the normal equations (algorithms/cg.py:83-176) and the
+ * ADMM-TV inner solve (cg.py:256-310, admm.py:470-646), float32 vectors with FP64
+ * inner products and step sizes kept on the device.
+ * scal: DEVICE double[4*nslices] per slice {rr, pHp, rr_new, tv}; work: DEVICE double
+ * scratch of ITOMO_VEC_WORK(nslices) entries.  n = elements per slice.
+ * Gradient images g hold 2 components per slice (nslices x 2 x npix x npix): d/drow
+ * and d/dcol of np.gradient (edge_order 1, unit spacing).
+ * ------------------------------------------------------------------------------- */
+#define ITOMO_VEC_WORK(nslices) ((int64_t)(nslices) * 771)
+/* g = G x (cg.py:62-63) and out = G^T g, its exact adjoint (cg.py:66-71). */
+int itomo_grad(const float *x, int npix, int nslices, float *g, void *stream);
+int itomo_grad_adjoint(const float *g, int npix, int nslices, float *out, void *stream);
+/* Hp += lambd1*p + mu*G^T g (g may be NULL: no TV terms); scal[1] = <p, Hp>. */
+int itomo_cg_hp(float *Hp, const float *p, const float *g, int npix, int nslices,
+                float lambd1, float mu, double *scal, double *work, void *stream);
+/* a = float(scal[0] / (scal[1] + 2^-52)); x += a p; r -= a Hp; scal[2] = <r, r>. */
+int itomo_cg_update(float *x, float *r, const float *p, const float *Hp, int64_t n,
+                    int nslices, double *scal, double *work, void *stream);
+/* p = r + float(scal[2] / (scal[0] [+ 2^-52])) p; then scal[0] = scal[2]. */
+int itomo_cg_direction(float *p, const float *r, int64_t n, int nslices, int with_eps,
+                       double *scal, void *stream);
+/* scal[slot] = <a, b> per slice. */
+int itomo_cg_dot(const float *a, const float *b, int64_t n, int nslices, int slot,
+                 double *scal, double *work, void *stream);
+/* r = atb + G^T v - (atax + lambd1*x + G^T gx) (cg.py:258); scal[0] = <r, r>. */
+int itomo_tv_residual(float *r, const float *atb, const float *atax, const float *x,
+                      const float *gx, const float *v, int npix, int nslices, float lambd1,
+                      double *scal, double *work, void *stream);
+/* v = mu*u - alpha over n elements (admm.py:532-534). */
+int itomo_tv_rhs(float *v, const float *u, const float *alpha, int64_t n, float mu,
+                 void *stream);
+/* u = soft(G x + alpha/mu, lambd2/mu); alpha += G x - u; scal[3] = sum (G x - u)^2
+ * (admm.py:460-461, 565-593, utils.py:214-216). */
+int itomo_admm_tv_update(const float *x, float *u, float *alpha, int npix, int nslices,
+                         float mu, float lambd2, double *scal, double *work, void *stream);
+
+/* ---------------------------------------------------------------------------------
  * FP64 reductions used by the monitoring / CG / ADMM / SPS scalars.
  * out (DEVICE double[1]) = sum_i a[i]*b[i] over n float32 entries, accumulated in FP64
  * in a fixed order.  b may equal a.  work: DEVICE scratch of ITOMO_DOT_WORK doubles.

@@ -45,7 +45,20 @@ _SIGS = {
     "itomo_mlem_back": ([_vp, _vp, _vp, _vp, _i, _vp, _vp], _i),
     "itomo_mlem_iterate": ([_vp, _vp, _vp, _vp, _vp, _i, _vp, _vp], _i),
     "itomo_dot_f64": ([_vp, _vp, _i64, _vp, _vp, _vp], _i),
+    "itomo_sps_weights": ([_vp, _vp, _i, _vp, _vp, _vp, _vp, _vp], _i),
+    "itomo_sps_iterate": ([_vp, _vp, _vp, _vp, _vp, _vp, _i, _vp, _vp], _i),
+    "itomo_grad": ([_vp, _i, _i, _vp, _vp], _i),
+    "itomo_grad_adjoint": ([_vp, _i, _i, _vp, _vp], _i),
+    "itomo_cg_hp": ([_vp, _vp, _vp, _i, _i, _f, _f, _vp, _vp, _vp], _i),
+    "itomo_cg_update": ([_vp, _vp, _vp, _vp, _i64, _i, _vp, _vp, _vp], _i),
+    "itomo_cg_direction": ([_vp, _vp, _i64, _i, _i, _vp, _vp], _i),
+    "itomo_cg_dot": ([_vp, _vp, _i64, _i, _i, _vp, _vp, _vp], _i),
+    "itomo_tv_residual": ([_vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _f, _vp, _vp, _vp], _i),
+    "itomo_tv_rhs": ([_vp, _vp, _vp, _i64, _f, _vp], _i),
+    "itomo_admm_tv_update": ([_vp, _vp, _vp, _i, _i, _f, _f, _vp, _vp, _vp], _i),
 }
+
+VEC_WORK_PER_SLICE = 771
 
 EXPORTS = tuple(_SIGS)
 

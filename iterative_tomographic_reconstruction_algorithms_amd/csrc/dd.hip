@@ -286,10 +286,10 @@ dd_back_kernel(const DdView* __restrict__ geom, int nang, int npix,
             if (r < npix && q < npix) {
                 const int64_t o = (int64_t)r * npix + q;
                 double q1 = 0.0, q2 = 0.0;
-                if (EPI == EPI_SIRT_BACK || EPI == EPI_MLEM_BACK)
-                    out[o] = epi_back<EPI>(acc[i][j], out[o], o, e, &q1, &q2);
+                if (epi_updates_x(EPI))
+                    out[o] = epi_back<EPI>(acc[i][j], out[o], o, z, e, &q1, &q2);
                 else
-                    out[o] = epi_back<EPI>(acc[i][j], 0.0f, o, e, &q1, &q2);
+                    out[o] = epi_back<EPI>(acc[i][j], 0.0f, o, z, e, &q1, &q2);
                 p1 += q1;
                 p2 += q2;
             }
@@ -426,7 +426,7 @@ dd_fwd_kernel(const DdView* __restrict__ geom, int nang, int npix,
                 }
             }
         }
-        out[soff + o] = epi_forward<EPI>(acc, o, soff, e, &p0);
+        out[soff + o] = epi_forward<EPI>(acc, o, soff, z, e, &p0);
     }
     if (epi_has_partials(EPI) && e.partials) {
         const int blk = blockIdx.y * gridDim.x + blockIdx.x;
@@ -496,6 +496,9 @@ int dispatch_fwd(const Plan& p, int epi, int ones, const float* image, const flo
     case EPI_MLEM_FWD:
         launch_fwd<ARITH, EPI_MLEM_FWD, false>(p, image, imageT, out, nslices, 0, e, st);
         return ITOMO_OK;
+    case EPI_SPS_FWD:
+        launch_fwd<ARITH, EPI_SPS_FWD, false>(p, image, imageT, out, nslices, 0, e, st);
+        return ITOMO_OK;
     }
     return set_error(ITOMO_EINVAL, "dd forward: unsupported epilogue %d", epi);
 }
@@ -517,6 +520,9 @@ int dispatch_back(const Plan& p, int epi, int ones, const float* sino, float* ou
         return ITOMO_OK;
     case EPI_MLEM_BACK:
         launch_back<ARITH, EPI_MLEM_BACK, false>(p, sino, out, nslices, 0, e, st);
+        return ITOMO_OK;
+    case EPI_SPS_BACK:
+        launch_back<ARITH, EPI_SPS_BACK, false>(p, sino, out, nslices, 0, e, st);
         return ITOMO_OK;
     }
     return set_error(ITOMO_EINVAL, "dd back: unsupported epilogue %d", epi);

@@ -13,11 +13,14 @@ enum Epi : int {
     EPI_SIRT_BACK = 3,  // x = x + C * acc; x<0 -> x*beta; part1 += dx^2; part2 += x^2
     EPI_MLEM_FWD = 4,   // ax = acc (0 -> 2^-52); out = b / ax; part0 += (ax - b)^2 (sir.py:270-271)
     EPI_MLEM_BACK = 5,  // x = x * (inv_d * acc); part1 += dx^2; part2 += x^2     (sir.py:283)
+    EPI_SPS_FWD = 6,    // out = (acc - b) * w; part0 += out^2                  (sir.py:434-441)
+    EPI_SPS_BACK = 7,   // x = x - acc / (den + 2^-52), quotient in FP64; part1, part2 (sir.py:535)
 };
 
 struct EpiArgs {
     const float* b;      // measured sinogram (slice stride = sino stride) -- forward epilogues
-    const float* w;      // R (nang x npix) / C or inv_d (npix x npix), shared by slices
+    const float* w;      // R (nang x npix) / C, inv_d (npix x npix) / SPS w, den
+    int64_t w_stride;    // elements between slices of w (0: one w shared by all slices)
     float pc_beta;       // SIRT back: < 0 disables the non-negativity step
     double* partials;    // [slice][block][3] FP64 partial sums, or nullptr
     int blocks_per_slice;
@@ -59,14 +62,19 @@ __device__ __forceinline__ void block_partials3(double a0, double a1, double a2,
 // Forward-side epilogue for output element index `o` (within one slice) of slice z.
 // Returns the value to store; writes the residual term to *p0.
 template <int EPI>
-__device__ __forceinline__ float epi_forward(float acc, int64_t o, int64_t slice_off,
+__device__ __forceinline__ float epi_forward(float acc, int64_t o, int64_t slice_off, int z,
                                              const EpiArgs& e, double* p0)
 {
-    if (EPI == EPI_SIRT_FWD) {
+    if (EPI == EPI_SPS_FWD) {
+        const float bv = e.b[slice_off + o];
+        const float dif = __fmul_rn(__fsub_rn(acc, bv), e.w[(int64_t)z * e.w_stride + o]);
+        *p0 = (double)dif * (double)dif;
+        return dif;
+    } else if (EPI == EPI_SIRT_FWD) {
         const float bv = e.b[slice_off + o];
         const float dif = __fsub_rn(bv, acc);
         *p0 = (double)dif * (double)dif;
-        return __fmul_rn(e.w[o], dif);
+        return __fmul_rn(e.w[(int64_t)z * e.w_stride + o], dif);
     } else if (EPI == EPI_MLEM_FWD) {
         const float bv = e.b[slice_off + o];
         float ax = acc;
@@ -84,18 +92,26 @@ __device__ __forceinline__ float epi_forward(float acc, int64_t o, int64_t slice
 
 // Back-side epilogue for pixel index `o` of slice z: `xo` is the current iterate value.
 template <int EPI>
-__device__ __forceinline__ float epi_back(float acc, float xo, int64_t o, const EpiArgs& e,
-                                          double* p1, double* p2)
+__device__ __forceinline__ float epi_back(float acc, float xo, int64_t o, int z,
+                                          const EpiArgs& e, double* p1, double* p2)
 {
-    if (EPI == EPI_SIRT_BACK) {
-        float xn = __fadd_rn(xo, __fmul_rn(e.w[o], acc));
+    const int64_t wo = (int64_t)z * e.w_stride + o;
+    if (EPI == EPI_SPS_BACK) {
+        const double q = (double)acc / ((double)e.w[wo] + 2.220446049250313e-16);
+        const float xn = (float)((double)xo - q);
+        const double dd = (double)xn - (double)xo;
+        *p1 = dd * dd;
+        *p2 = (double)xo * (double)xo;
+        return xn;
+    } else if (EPI == EPI_SIRT_BACK) {
+        float xn = __fadd_rn(xo, __fmul_rn(e.w[wo], acc));
         if (e.pc_beta >= 0.0f && xn < 0.0f) xn = __fmul_rn(xn, e.pc_beta);
         const double dd = (double)xn - (double)xo;
         *p1 = dd * dd;
         *p2 = (double)xo * (double)xo;
         return xn;
     } else if (EPI == EPI_MLEM_BACK) {
-        const float xn = __fmul_rn(xo, __fmul_rn(e.w[o], acc));
+        const float xn = __fmul_rn(xo, __fmul_rn(e.w[wo], acc));
         const double dd = (double)xn - (double)xo;
         *p1 = dd * dd;
         *p2 = (double)xo * (double)xo;
@@ -111,7 +127,12 @@ __device__ __forceinline__ float epi_back(float acc, float xo, int64_t o, const 
 constexpr bool epi_has_partials(int epi)
 {
     return epi == EPI_SIRT_FWD || epi == EPI_SIRT_BACK || epi == EPI_MLEM_FWD ||
-           epi == EPI_MLEM_BACK;
+           epi == EPI_MLEM_BACK || epi == EPI_SPS_FWD || epi == EPI_SPS_BACK;
+}
+
+constexpr bool epi_updates_x(int epi)
+{
+    return epi == EPI_SIRT_BACK || epi == EPI_MLEM_BACK || epi == EPI_SPS_BACK;
 }
 
 // host helpers (iter.hip)

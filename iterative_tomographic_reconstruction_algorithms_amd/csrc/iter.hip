@@ -74,8 +74,8 @@ back_epi_kernel(const float* __restrict__ acc, float* __restrict__ x, EpiArgs e,
         const int64_t o = base + (int64_t)k * 256 + threadIdx.x;
         if (o < n) {
             double q1 = 0.0, q2 = 0.0;
-            const float xo = (EPI == EPI_SIRT_BACK || EPI == EPI_MLEM_BACK) ? x[o] : 0.0f;
-            x[o] = epi_back<EPI>(acc[o], xo, o, e, &q1, &q2);
+            const float xo = epi_updates_x(EPI) ? x[o] : 0.0f;
+            x[o] = epi_back<EPI>(acc[o], xo, o, z, e, &q1, &q2);
             p1 += q1;
             p2 += q2;
         }
@@ -92,9 +92,11 @@ int elementwise_back_epilogue(int epi, const float* acc_img, float* x, const Epi
     ee.blocks_per_slice = (int)((n + kEpiChunk - 1) / kEpiChunk);
     dim3 grid(ee.blocks_per_slice, nslices);
     switch (epi) {
+    case EPI_STORE: back_epi_kernel<EPI_STORE><<<grid, 256, 0, st>>>(acc_img, x, ee, n); break;
     case EPI_RECIP: back_epi_kernel<EPI_RECIP><<<grid, 256, 0, st>>>(acc_img, x, ee, n); break;
     case EPI_SIRT_BACK: back_epi_kernel<EPI_SIRT_BACK><<<grid, 256, 0, st>>>(acc_img, x, ee, n); break;
     case EPI_MLEM_BACK: back_epi_kernel<EPI_MLEM_BACK><<<grid, 256, 0, st>>>(acc_img, x, ee, n); break;
+    case EPI_SPS_BACK: back_epi_kernel<EPI_SPS_BACK><<<grid, 256, 0, st>>>(acc_img, x, ee, n); break;
     default: return set_error(ITOMO_EINVAL, "elementwise epilogue %d unsupported", epi);
     }
     return check_hip(hipGetLastError(), "back epilogue launch");
@@ -131,6 +133,31 @@ int rd_forward_epi(const Plan& p, int epi, int ones, const float* image, float* 
 int rd_back_scatter(const Plan& p, int ones, const float* sino, float* image, int nslices,
                     hipStream_t st);
 
+int projector_forward_epi_any(const Plan& p, int epi, int ones, const float* image, float* out,
+                              int nslices, const EpiArgs& e, hipStream_t st, int* bps)
+{
+    if (p.projector == ITOMO_PROJ_DD)
+        return dd_forward_epi(p, epi, ones, image, out, nslices, 0, e, st, bps);
+    return rd_forward_epi(p, epi, ones, image, out, nslices, 0, e, st, bps);
+}
+
+// Back projection followed by a back-side epilogue writing `x`.  The ray-driven path
+// scatters into plan scratch first.
+int projector_back_epi_any(const Plan& p, int epi, int ones, const float* sino, float* x,
+                           int nslices, const EpiArgs& e, hipStream_t st, int* bps)
+{
+    if (p.projector == ITOMO_PROJ_DD)
+        return dd_back_epi(p, epi, ones, sino, x, nslices, 0, e, st, bps);
+    Scratch s;
+    int rc = get_scratch(p, nslices, &s, st);
+    if (rc) return rc;
+    const int64_t n = (int64_t)p.npix * p.npix;
+    rc = rd_back_scatter(p, ones, sino, s.img, nslices, st);
+    if (rc) return rc;
+    if (bps) *bps = (int)((n + kEpiChunk - 1) / kEpiChunk);
+    return elementwise_back_epilogue(epi, s.img, x, e, n, nslices, st);
+}
+
 }  // namespace itomo
 
 using namespace itomo;
@@ -140,23 +167,13 @@ namespace {
 int projector_forward_epi(const Plan& p, int epi, int ones, const float* image, float* out,
                           int nslices, const EpiArgs& e, hipStream_t st, int* bps)
 {
-    if (p.projector == ITOMO_PROJ_DD)
-        return dd_forward_epi(p, epi, ones, image, out, nslices, 0, e, st, bps);
-    return rd_forward_epi(p, epi, ones, image, out, nslices, 0, e, st, bps);
+    return projector_forward_epi_any(p, epi, ones, image, out, nslices, e, st, bps);
 }
 
-// Back projection followed by a back-side epilogue writing `x`.  The ray-driven path
-// scatters into plan scratch `tmp` first.
 int projector_back_epi(const Plan& p, int epi, int ones, const float* sino, float* x,
-                       int nslices, const EpiArgs& e, float* tmp, hipStream_t st, int* bps)
+                       int nslices, const EpiArgs& e, float*, hipStream_t st, int* bps)
 {
-    if (p.projector == ITOMO_PROJ_DD)
-        return dd_back_epi(p, epi, ones, sino, x, nslices, 0, e, st, bps);
-    const int64_t n = (int64_t)p.npix * p.npix;
-    int rc = rd_back_scatter(p, ones, sino, tmp, nslices, st);
-    if (rc) return rc;
-    if (bps) *bps = (int)((n + kEpiChunk - 1) / kEpiChunk);
-    return elementwise_back_epilogue(epi, tmp, x, e, n, nslices, st);
+    return projector_back_epi_any(p, epi, ones, sino, x, nslices, e, st, bps);
 }
 
 using IterScratch = Scratch;

@@ -213,7 +213,7 @@ rd_fwd_kernel(const RdView* __restrict__ geom, int npix, const float* __restrict
                 acc = rd_walk<ARITH, 0, ONES>(acc, image, nullptr, npix, p1, p2);
             }
         }
-        out[soff + o] = epi_forward<EPI>(acc, o, soff, e, &p0);
+        out[soff + o] = epi_forward<EPI>(acc, o, soff, z, e, &p0);
     }
     if (epi_has_partials(EPI) && e.partials) {
         const int blk = blockIdx.y * gridDim.x + blockIdx.x;
@@ -278,6 +278,9 @@ int dispatch_rd_fwd(const Plan& p, int epi, int ones, const float* image, float*
         return ITOMO_OK;
     case EPI_MLEM_FWD:
         launch_rd_fwd<ARITH, EPI_MLEM_FWD, false>(p, image, out, nslices, 0, e, st);
+        return ITOMO_OK;
+    case EPI_SPS_FWD:
+        launch_rd_fwd<ARITH, EPI_SPS_FWD, false>(p, image, out, nslices, 0, e, st);
         return ITOMO_OK;
     }
     return set_error(ITOMO_EINVAL, "rd forward: unsupported epilogue %d", epi);

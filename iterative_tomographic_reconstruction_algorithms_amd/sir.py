@@ -1,0 +1,62 @@
+"""Drop-in for the MLEM and SPS functions of the reference's algorithms/sir.py
+(sir.py:192-679), running on the GPU.  SPS penalties (huber, tikhonov, haar) are outside
+this path and raise NotImplementedError."""
+import numpy as np
+import torch
+
+from . import solvers
+from .class_projectors_radon import projectors
+from .params import PROJECTOR_OPER
+from .solvers import _dev
+
+myfloat = np.float32
+
+
+def _out(t, like):
+    return t if isinstance(like, torch.Tensor) else t.cpu().numpy()
+
+
+def forward(tp, x):         # sir.py:192-193
+    return tp.A(x)
+
+
+def backward(tp, b):        # sir.py:198-199
+    return tp.At(b)
+
+
+def em(x, b, tp, param):
+    """sir.py:204-324: returns (x, info) with info rows [err1, err2]."""
+    n_iter = int(param.n_iter) if param.n_iter is not None and param.n_iter < 1e19 else 200
+    eps = -1.0 if param.eps is None else float(param.eps)
+    xd = _dev(x).clone()
+    plan = tp.plan(xd.shape[-1])
+    xd, info, _ = solvers.mlem(plan, xd, _dev(b), n_iter, eps=eps)
+    xd = xd.reshape(x.shape) if isinstance(x, torch.Tensor) else xd.reshape(np.shape(x))
+    return _out(xd, x), np.array(info)
+
+
+def sps(x, b, tp, param):
+    """sir.py:348-600 without penalty: returns (x, info) with info rows [err, 0]."""
+    if getattr(param, "reg", None) not in (None, "none") and getattr(param, "reg_cost", 0.0):
+        raise NotImplementedError("SPS penalties are outside the GPU path")
+    n_iter = int(param.n_iter) if param.n_iter is not None and param.n_iter < 1e19 else 200
+    xd = _dev(x).clone()
+    plan = tp.plan(xd.shape[-1])
+    xd, err = solvers.sps(plan, xd, _dev(b), n_iter)
+    xd = xd.reshape(x.shape) if isinstance(x, torch.Tensor) else xd.reshape(np.shape(x))
+    return _out(xd, x), np.stack([err, np.zeros_like(err)], axis=1)
+
+
+def sir(b, a, param):
+    """sir.py:613-679 for the radon projectors: x starts from ones (sir.py:644)."""
+    oper = PROJECTOR_OPER.get(param.projector, param.projector)
+    n = param.npix_op
+    tp = projectors(n, a, oper=oper)
+    b = np.asarray(b, dtype=myfloat)
+    b = b.reshape(-1, b.shape[-2], b.shape[-1])
+    x = np.ones((b.shape[0], n, n), dtype=myfloat)
+    if param.algorithm == "em":
+        return em(x, b, tp, param)
+    if param.algorithm == "sps":
+        return sps(x, b, tp, param)
+    raise ValueError("algorithm %r" % param.algorithm)

@@ -65,3 +65,106 @@ def mlem(x0, b, A, At, n_iter, dtype=np.float32):
         xa = At(q).astype(dtype)
         x = (x * (inv_d * xa).astype(dtype)).astype(dtype)   # sir.py:283
     return x, np.array(err)
+
+
+def sps(x0, b, A, At, n_iter, dtype=np.float32):
+    """sir.py:348-540 with beta = 0: w = exp(-b/max b); den = At(w*A(1));
+    x <- x - At(w*(A x - b)) / (den + spacing(1)), the quotient evaluated in float64
+    (the GPU path's FP64 denominator).  Returns (x, err) with err[k] = sum(w(Ax_k-b))^2/sum b^2."""
+    n = x0.shape[-1]
+    b = b.astype(dtype)
+    w = np.exp(-b / np.max(b)).astype(dtype)
+    den = At((w * A(np.ones((n, n), dtype=dtype))).astype(dtype)).astype(dtype)
+    x = x0.astype(dtype).copy()
+    bden = np.sum(b.astype(np.float64) ** 2)
+    err = []
+    for _ in range(n_iter):
+        bd = ((A(x).astype(dtype) - b) * w).astype(dtype)
+        err.append(np.sum(bd.astype(np.float64) ** 2) / bden)
+        num = At(bd).astype(dtype)
+        q = num.astype(np.float64) / (den.astype(np.float64) + np.spacing(1))
+        x = (x.astype(np.float64) - q).astype(dtype)
+    return x, np.array(err)
+
+
+def gradient(m):
+    """cg.py:62-63 (np.gradient, a library routine)."""
+    return np.gradient(m)
+
+
+def gradient_adjoint(g):
+    """cg.py:66-71 as written: dot(g1, Gx) + dot(Gy, g0) with dense difference matrices."""
+    nx, ny = g[0].shape
+    Gy = np.gradient(np.diag(np.ones(nx)))[1]
+    Gx = np.gradient(np.diag(np.ones(ny)))[0]
+    return np.dot(g[1], Gx) + np.dot(Gy, g[0])
+
+
+def cg(x0, b, A, At, n_iter, dtype=np.float32, lambd1=0.0, mu=0.0, v=None, tv=False,
+       with_eps=True):
+    """cg.py:83-160 (tv=False) and cg.py:256-306 (tv=True): CG on
+    (At A + lambd1 I + mu Gt G) x = At b (+ Gt v), vectors in `dtype`, inner products and
+    step sizes in float64 (the GPU path's FP64 scalars; numpy-1 rounds alpha and beta to
+    the vector dtype before scaling).  Returns (x, rr) with rr[k] = <r_{k+1}, r_{k+1}>."""
+    dt = dtype
+    x = x0.astype(dt).copy()
+
+    def H(p):
+        h = At(A(p).astype(dt)).astype(dt)
+        if tv:
+            gp = [g.astype(dt) for g in gradient(p)]
+            h = (h + dt(lambd1) * p).astype(dt)
+            h = (h + dt(mu) * gradient_adjoint(gp).astype(dt)).astype(dt)
+        return h
+
+    if tv:
+        gx = [g.astype(dt) for g in gradient(x)]
+        lhs = (At(b).astype(dt) + gradient_adjoint(v).astype(dt)).astype(dt)
+        rhs = ((At(A(x).astype(dt)).astype(dt) + dt(lambd1) * x).astype(dt)
+               + gradient_adjoint(gx).astype(dt)).astype(dt)
+        r = (lhs - rhs).astype(dt)
+    else:
+        r = (At(b).astype(dt) - At(A(x).astype(dt)).astype(dt)).astype(dt)
+    p = r.copy()
+    rr = float(np.sum(r.astype(np.float64) ** 2))
+    out = []
+    eps = np.spacing(1)
+    for _ in range(n_iter):
+        hp = H(p)
+        php = float(np.sum(p.astype(np.float64) * hp.astype(np.float64)))
+        a = dt(rr / (php + eps))
+        x = (x + (a * p).astype(dt)).astype(dt)
+        r = (r - (a * hp).astype(dt)).astype(dt)
+        rn = float(np.sum(r.astype(np.float64) ** 2))
+        out.append(rn)
+        bta = dt(rn / (rr + (eps if with_eps else 0.0)))
+        p = (r + (bta * p).astype(dt)).astype(dt)
+        rr = rn
+    return x, np.array(out)
+
+
+def soft(y, t):
+    """utils.py:214-216 (hard_thres is a soft threshold)."""
+    return np.maximum(0, y - t) - np.maximum(0, -y - t)
+
+
+def admm_tv(x0, b, A, At, n_iter, n_inner=4, lambd1=1.0, lambd2=1.0, mu=1.0, dtype=np.float32):
+    """admm.py:470-604 for one slice: inner CG (cg.py:256-310) then
+    u = soft(G x + alpha/mu, lambd2/mu), alpha += G x - u.  Returns (x, err) with
+    err[k] = ||G x - u|| after outer step k."""
+    dt = dtype
+    n = x0.shape[-1]
+    x = x0.astype(dt).copy()
+    u = np.zeros((2, n, n), dtype=dt)
+    al = np.zeros((2, n, n), dtype=dt)
+    err = []
+    for _ in range(n_iter):
+        v = (dt(mu) * u - al).astype(dt)
+        x, _ = cg(x, b, A, At, n_inner, dtype=dt, lambd1=lambd1, mu=mu, v=v, tv=True,
+                  with_eps=False)
+        g = np.array([gg.astype(dt) for gg in gradient(x)])
+        u = soft((g + (al / dt(mu)).astype(dt)).astype(dt), dt(lambd2) / dt(mu)).astype(dt)
+        r = (g - u).astype(dt)
+        al = (al + r).astype(dt)
+        err.append(np.sqrt(np.sum(r.astype(np.float64) ** 2)))
+    return x, np.array(err)

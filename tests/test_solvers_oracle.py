@@ -9,6 +9,7 @@ the mathematics fixes, so a dropped or misplaced weight fails:
   sum(A x_{k+1}) = sum(b) when every bin is covered.
 """
 import numpy as np
+import pytest
 
 from oracle import radon as O
 from oracle import solvers as S
@@ -59,3 +60,66 @@ def test_mlem_positive_and_counts_preserved_on_rd():
     x, err = S.mlem(np.ones((n, n)), p, A, At, 5, dtype=np.float64)
     assert (x > 0).sum() > 0 and (x >= 0).all()
     assert err[-1] < err[0]
+
+
+def test_sps_one_step_solves_diagonal_system():
+    rng = np.random.default_rng(2)
+    a = rng.uniform(0.5, 2.0, (4, 4))
+    b = rng.uniform(0.1, 1.0, (4, 4))
+    A = lambda x: a * x
+    At = lambda y: a * y
+    x, err = S.sps(rng.uniform(0, 1, (4, 4)), b, A, At, 1, dtype=np.float64)
+    np.testing.assert_allclose(x, b / a, rtol=1e-12)
+
+
+def _dense(n, v, seed):
+    rng = np.random.default_rng(seed)
+    M = rng.standard_normal((v * n, n * n))
+    A = lambda x: (M @ x.reshape(-1)).reshape(v, n)
+    At = lambda y: (M.T @ y.reshape(-1)).reshape(n, n)
+    return M, A, At
+
+
+def test_cg_reaches_least_squares_solution():
+    n, v = 4, 6
+    M, A, At = _dense(n, v, 3)
+    b = np.random.default_rng(4).standard_normal((v, n))
+    x, rr = S.cg(np.zeros((n, n)), b, A, At, n * n + 4, dtype=np.float64)
+    ref = np.linalg.lstsq(M, b.reshape(-1), rcond=None)[0]
+    np.testing.assert_allclose(x.reshape(-1), ref, rtol=1e-6, atol=1e-8)
+
+
+def test_tv_cg_solves_the_regularised_normal_equations():
+    n, v = 4, 3
+    M, A, At = _dense(n, v, 5)
+    rng = np.random.default_rng(6)
+    b = rng.standard_normal((v, n))
+    vv = rng.standard_normal((2, n, n))
+    lambd1, mu = 0.3, 0.7
+    # dense G from np.gradient applied to unit images
+    G = np.zeros((2 * n * n, n * n))
+    for k in range(n * n):
+        e = np.zeros(n * n)
+        e[k] = 1.0
+        g = np.gradient(e.reshape(n, n))
+        G[:, k] = np.concatenate([g[0].reshape(-1), g[1].reshape(-1)])
+    H = M.T @ M + lambd1 * np.eye(n * n) + mu * G.T @ G
+    rhs = M.T @ b.reshape(-1) + G.T @ vv.reshape(-1)
+    x, _ = S.cg(np.zeros((n, n)), b, A, At, 3 * n * n, dtype=np.float64, lambd1=lambd1, mu=mu,
+                v=vv, tv=True, with_eps=False)
+    np.testing.assert_allclose(x.reshape(-1), np.linalg.solve(H, rhs), rtol=1e-6, atol=1e-8)
+
+
+@pytest.mark.parametrize("n", [2, 3, 4, 7])
+def test_gradient_adjoint_is_exact(n):
+    rng = np.random.default_rng(n)
+    x = rng.standard_normal((n, n))
+    g = rng.standard_normal((2, n, n))
+    lhs = sum(np.vdot(a, b) for a, b in zip(S.gradient(x), g))
+    rhs = np.vdot(x, S.gradient_adjoint(g))
+    assert abs(lhs - rhs) < 1e-12 * (1 + abs(lhs))
+
+
+def test_soft_threshold_closed_form():
+    y = np.linspace(-3, 3, 61)
+    np.testing.assert_allclose(S.soft(y, 1.25), np.sign(y) * np.maximum(np.abs(y) - 1.25, 0))
