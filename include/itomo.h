@@ -130,6 +130,37 @@ int itomo_mlem_iterate(const itomo_plan *plan, float *x, const float *b,
                        void *stream);
 
 /* ---------------------------------------------------------------------------------
+ * Batched SIRT over several geometries in one persistent launch (no reference
+ * counterpart: the reference fans slices out to a multiprocessing.Pool, art.py:289-295).
+ * Member m is a distance-driven FAST plan with nslices[m] stacked slices and its own
+ * x, b, R, C, work (as for itomo_sirt_iterate) and stats (DEVICE double[3*nslices[m]] or
+ * NULL).  One iterate call = transpose + persistent forward + persistent back +
+ * finalize; results equal itomo_sirt_iterate on each member bit for bit.  Creating a
+ * batch allocates its work tables and scratch; iterate allocates nothing.
+ * ------------------------------------------------------------------------------- */
+typedef struct itomo_batch itomo_batch;
+int itomo_batch_create(itomo_batch **batch, int nmembers, const itomo_plan *const *plans,
+                       const int *nslices, float *const *x, const float *const *b,
+                       const float *const *R, const float *const *C, float *const *work,
+                       double *const *stats);
+int itomo_batch_sirt_iterate(itomo_batch *batch, float pc_beta, void *stream);
+int itomo_batch_destroy(itomo_batch *batch);
+
+/* ---------------------------------------------------------------------------------
+ * hipGraph capture of `iters` SIRT iterations (single plan or batch); the graph is
+ * replayed with itomo_graph_launch on any stream.  Capture synchronizes once with the
+ * work already queued on `stream`.  The graph reuses the arrays it was captured with.
+ * ------------------------------------------------------------------------------- */
+typedef struct itomo_graph itomo_graph;
+int itomo_graph_sirt(itomo_graph **graph, const itomo_plan *plan, float *x, const float *b,
+                     const float *R, const float *C, float *work, int nslices,
+                     float pc_beta, double *stats, int iters, void *stream);
+int itomo_graph_batch_sirt(itomo_graph **graph, itomo_batch *batch, float pc_beta,
+                           int iters, void *stream);
+int itomo_graph_launch(itomo_graph *graph, void *stream);
+int itomo_graph_destroy(itomo_graph *graph);
+
+/* ---------------------------------------------------------------------------------
  * SPS without penalty (algorithms/sir.py:348-540, beta = 0):
  *   setup: w = exp(-b / max(b)) over all slices, den = A^T(w * A(1)) per slice
  *   step:  work = (A x - b) * w;  x <- x - A^T(work) / (den + 2^-52), the quotient in FP64
@@ -178,6 +209,18 @@ int itomo_tv_rhs(float *v, const float *u, const float *alpha, int64_t n, float 
  * (admm.py:460-461, 565-593, utils.py:214-216). */
 int itomo_admm_tv_update(const float *x, float *u, float *alpha, int npix, int nslices,
                          float mu, float lambd2, double *scal, double *work, void *stream);
+
+/* ---------------------------------------------------------------------------------
+ * Sinogram filtering for filtered back projection (filters.py:35-113, dpc = False).
+ * itomo_filter_taps (HOST): taps[k], k < npix, of the real even impulse response of the
+ *   reference's zero-padded FFT filter; ftype 0 ramp, 1 shepp-logan, 2 cosine,
+ *   3 hamming, 4 hanning, 5 none.  npix <= 4096.
+ * itomo_filter_rows: out[r][n] = sum_m in[r][m] * taps[|n-m|] (DEVICE arrays, FP64
+ *   accumulation) -- identical to the reference's FFT filtering for n < npix.
+ * ------------------------------------------------------------------------------- */
+int itomo_filter_taps(int npix, int ftype, double *taps_host);
+int itomo_filter_rows(const float *in, float *out, int nrows, int npix, const double *taps,
+                      void *stream);
 
 /* ---------------------------------------------------------------------------------
  * FP64 reductions used by the monitoring / CG / ADMM / SPS scalars.

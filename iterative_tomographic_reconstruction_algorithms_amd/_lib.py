@@ -56,7 +56,18 @@ _SIGS = {
     "itomo_tv_residual": ([_vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _f, _vp, _vp, _vp], _i),
     "itomo_tv_rhs": ([_vp, _vp, _vp, _i64, _f, _vp], _i),
     "itomo_admm_tv_update": ([_vp, _vp, _vp, _i, _i, _f, _f, _vp, _vp, _vp], _i),
+    "itomo_batch_create": ([ctypes.POINTER(_vp), _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp], _i),
+    "itomo_batch_sirt_iterate": ([_vp, _f, _vp], _i),
+    "itomo_batch_destroy": ([_vp], _i),
+    "itomo_graph_sirt": ([ctypes.POINTER(_vp), _vp, _vp, _vp, _vp, _vp, _vp, _i, _f, _vp, _i, _vp], _i),
+    "itomo_graph_batch_sirt": ([ctypes.POINTER(_vp), _vp, _f, _i, _vp], _i),
+    "itomo_graph_launch": ([_vp, _vp], _i),
+    "itomo_graph_destroy": ([_vp], _i),
+    "itomo_filter_taps": ([_i, _i, _vp], _i),
+    "itomo_filter_rows": ([_vp, _vp, _i, _i, _vp, _vp], _i),
 }
+
+FILTERS = {"ramp": 0, "shepp-logan": 1, "cosine": 2, "hamming": 3, "hanning": 4, "none": 5}
 
 VEC_WORK_PER_SLICE = 771
 

@@ -11,7 +11,7 @@ import numpy as np
 import torch
 
 from . import _dropin as D
-from .core import Plan
+from .core import Plan, filter_sino
 
 myfloat = np.float32
 
@@ -70,3 +70,18 @@ class projectors:
         p = self.plan(x.shape[-1])
         t, was_np = D.to_device(x)
         return D.from_device(p.back(t), was_np)
+
+    # Filtered backprojection (class_projectors_radon.py:86-110).  The reference raises
+    # NameError (`dpc` is undefined at line 92) and back-projects the unfiltered array;
+    # this follows the steps it spells out with the filtered array and dpc = False:
+    # reverse the bins, filter, back-project, scale by pi/(2 nang), rotate by 180 degrees.
+    def fbp(self, x):
+        x = self._as_f32(x)
+        p = self.plan(x.shape[-1])
+        t, was_np = D.to_device(x)
+        x0 = torch.flip(t, dims=(-1,)).contiguous()
+        x0 = filter_sino(x0, ftype=self.filt)
+        reco = p.back(x0)
+        reco.mul_(np.float32(np.pi / (2.0 * self.nang)))
+        reco = torch.flip(reco, dims=(-2, -1)).contiguous()
+        return D.from_device(reco, was_np)

@@ -224,6 +224,32 @@ class Plan:
         return x
 
 
+_taps = {}
+
+
+def filter_taps(npix, ftype="ramp"):
+    """FP64 taps of the reference's sinogram filter (filters.py:35-113) on the GPU."""
+    key = (int(npix), ftype)
+    t = _taps.get(key)
+    if t is None:
+        h = np.empty(npix, dtype=np.float64)
+        L.check(L.lib().itomo_filter_taps(int(npix), L.FILTERS[ftype],
+                                          h.ctypes.data_as(ctypes.c_void_p)), "filter_taps")
+        t = _taps[key] = torch.from_numpy(h).to(device())
+    return t
+
+
+def filter_sino(sino, ftype="ramp"):
+    """filters.filter_proj on the GPU: each row convolved with the filter taps."""
+    _check_dev(sino, "sino")
+    npix = sino.shape[-1]
+    taps = filter_taps(npix, ftype)
+    out = torch.empty_like(sino)
+    L.check(L.lib().itomo_filter_rows(_p(sino), _p(out), sino.numel() // npix, npix, _p(taps),
+                                      stream_handle()), "filter_rows")
+    return out
+
+
 _dot_work = {}
 
 

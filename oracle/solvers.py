@@ -168,3 +168,41 @@ def admm_tv(x0, b, A, At, n_iter, n_inner=4, lambd1=1.0, lambd2=1.0, mu=1.0, dty
         al = (al + r).astype(dt)
         err.append(np.sqrt(np.sum(r.astype(np.float64) ** 2)))
     return x, np.array(err)
+
+
+def calc_filter(nfreq, ftype="ramp"):
+    """filters.py:35-76 with dpc = False."""
+    if ftype == "none":
+        return np.ones(2 * nfreq)
+    filtarr = 2 * np.arange(nfreq + 1) / np.float32(2 * nfreq)
+    w = 2 * np.pi * np.arange(nfreq + 1) / np.float32(2 * nfreq)
+    d = 1.0
+    if ftype == "shepp-logan":
+        filtarr[1:] *= np.sin(w[1:]) / (2.0 * d * 2.0 * d * w[1:])
+    elif ftype == "cosine":
+        filtarr[1:] *= np.cos(w[1:]) / (2.0 * d * w[1:])
+    elif ftype == "hamming":
+        filtarr[1:] *= (0.54 + 0.46 * np.cos(w[1:]) / d)
+    elif ftype == "hanning":
+        filtarr[1:] *= (1.0 + np.cos(w[1:] / d) / 2.0)
+    return np.concatenate((filtarr, filtarr[nfreq - 1:0:-1]), axis=0)
+
+
+def filter_proj(sino, ftype="ramp"):
+    """filters.py:89-113: zero-pad each row to 2*nfreq, multiply by the filter in Fourier
+    space, keep the first npix samples (float64 arithmetic, float32 result)."""
+    nang, npix = sino.shape
+    nfreq = 2 * int(2 ** (int(np.ceil(np.log2(npix)))))
+    filtarr = calc_filter(nfreq, ftype)
+    padded = np.concatenate((sino, np.zeros((nang, 2 * nfreq - npix))), axis=1)
+    out = np.real(np.fft.ifft(np.fft.fft(padded, axis=1) * filtarr, axis=1))
+    return out[:, :npix].astype(np.float32)
+
+
+def fbp(sino, back, nang, ftype="ramp"):
+    """class_projectors_radon.py:86-110 with its two defects repaired (dpc = False, the
+    filtered array is the one back-projected)."""
+    x0 = sino[:, ::-1].copy()
+    x0 = filter_proj(x0, ftype)
+    reco = back(x0).astype(np.float32) * np.float32(np.pi / (2.0 * nang))
+    return reco[::-1, ::-1].copy()
