@@ -18,6 +18,7 @@ namespace itomo {
 
 struct BatchMember {
     const DdView* geom;
+    const float* lstar;
     int nang, npix, nslices;
     float* x;
     const float* b;
@@ -96,7 +97,7 @@ batch_fwd_kernel(const BatchMember* __restrict__ mem, const BatchItem* __restric
         e.blocks_per_slice = m.bps_f;
         const int nbx = (m.npix + kDdFwdBlock - 1) / kDdFwdBlock;
         dd_fwd_block<ITOMO_ARITH_FAST, EPI_SIRT_FWD, false>(
-            m.geom, m.nang, m.npix, m.x, m.xT, m.work, (int64_t)m.npix * m.npix,
+            m.geom, m.lstar, m.nang, m.npix, m.x, m.xT, m.work, (int64_t)m.npix * m.npix,
             (int64_t)m.nang * m.npix, 0, e, w.a, w.b, w.z, w.b * nbx + w.a);
     }
 }
@@ -117,7 +118,7 @@ batch_back_kernel(const BatchMember* __restrict__ mem, const BatchItem* __restri
         e.blocks_per_slice = m.bps_b;
         const int nbx = (m.npix + kDdBackTile - 1) / kDdBackTile;
         dd_back_tile<ITOMO_ARITH_FAST, EPI_SIRT_BACK, false>(
-            m.geom, m.nang, m.npix, m.work, m.x, (int64_t)m.nang * m.npix,
+            m.geom, m.lstar, m.nang, m.npix, m.work, m.x, (int64_t)m.nang * m.npix,
             (int64_t)m.npix * m.npix, 0, e, w.a, w.b, w.z, w.b * nbx + w.a);
     }
 }
@@ -200,6 +201,7 @@ int itomo_batch_create(itomo_batch** out, int nmembers, const itomo_plan* const*
         }
         BatchMember bm{};
         bm.geom = (const DdView*)p->d_geom;
+        bm.lstar = p->d_lstar;
         bm.nang = p->nang;
         bm.npix = p->npix;
         bm.nslices = nslices[m];

@@ -4,10 +4,11 @@
 // detector-bin boundaries and sweeps the merged list; each segment between consecutive
 // keys that lies inside pixel q and bin d moves weight |segment| between them, and the
 // segment that ends at the first key of index N is dropped (radon_dd.c:176-177).
-// Here the same weights are evaluated in gather form: w(q, d) = min(P[q+1], D[d+1]) -
-// max(P[q], D[d]) when positive, and 0 for the one pair per line whose right end is
-// E = min(P[N], D[N]).  Keys are formed with exactly the float operations of the
-// reference build, so every weight is bit-identical to the reference's.
+// Here the same weights are evaluated in gather form: w(q, d) = min(P[q+1], D[d+1], L*)
+// - max(P[q], D[d]) when positive, where L* is the start of the line's dropped segment
+// (one float per view and line, tabulated when the plan is made: dd_line_end).  Keys
+// are formed with exactly the float operations of the reference build, so every weight
+// is bit-identical to the reference's.
 //
 // Accumulation order is also the reference's, so results are bitwise reproducible:
 //  * forward: one thread per (view, bin) walks image lines in increasing order and,
@@ -24,23 +25,23 @@ namespace itomo {
 
 template <int ARITH, int EPI, bool ONES>
 __global__ void __launch_bounds__(256)
-dd_back_kernel(const DdView* __restrict__ geom, int nang, int npix,
-               const float* __restrict__ sino, float* __restrict__ out,
+dd_back_kernel(const DdView* __restrict__ geom, const float* __restrict__ lstar, int nang,
+               int npix, const float* __restrict__ sino, float* __restrict__ out,
                int64_t sino_stride, int64_t img_stride, int accumulate, EpiArgs e)
 {
-    dd_back_tile<ARITH, EPI, ONES>(geom, nang, npix, sino, out, sino_stride, img_stride,
+    dd_back_tile<ARITH, EPI, ONES>(geom, lstar, nang, npix, sino, out, sino_stride, img_stride,
                                    accumulate, e, blockIdx.x, blockIdx.y, blockIdx.z,
                                    blockIdx.y * gridDim.x + blockIdx.x);
 }
 
 template <int ARITH, int EPI, bool ONES>
 __global__ void __launch_bounds__(kDdFwdBlock)
-dd_fwd_kernel(const DdView* __restrict__ geom, int nang, int npix,
-              const float* __restrict__ image, const float* __restrict__ imageT,
+dd_fwd_kernel(const DdView* __restrict__ geom, const float* __restrict__ lstar, int nang,
+              int npix, const float* __restrict__ image, const float* __restrict__ imageT,
               float* __restrict__ out, int64_t img_stride, int64_t sino_stride,
               int accumulate, EpiArgs e)
 {
-    dd_fwd_block<ARITH, EPI, ONES>(geom, nang, npix, image, imageT, out, img_stride,
+    dd_fwd_block<ARITH, EPI, ONES>(geom, lstar, nang, npix, image, imageT, out, img_stride,
                                    sino_stride, accumulate, e, blockIdx.x, blockIdx.y,
                                    blockIdx.z, blockIdx.y * gridDim.x + blockIdx.x);
 }
@@ -52,6 +53,19 @@ transpose_kernel(const float* __restrict__ in, float* __restrict__ out, int npix
     transpose_tile(in, out, npix, stride, blockIdx.x, blockIdx.y, blockIdx.z);
 }
 
+// lstar[v * npix + i] = L* of line i in view v (0 for views without a sweep).
+template <int ARITH>
+__global__ void __launch_bounds__(256)
+line_end_kernel(const DdView* __restrict__ geom, int nang, int npix, float* __restrict__ lstar)
+{
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= (int64_t)nang * npix) return;
+    const int v = (int)(k / npix), i = (int)(k - (int64_t)v * npix);
+    const DdView g = geom[v];
+    const bool sweep = g.kind == DD_SWEEP_ROWS || g.kind == DD_SWEEP_COLS;
+    lstar[k] = sweep ? dd_line_end<ARITH>(g, npix, (int)(npix * 0.5), i) : 0.0f;
+}
+
 namespace {
 
 template <int ARITH, int EPI, bool ONES>
@@ -60,7 +74,7 @@ void launch_fwd(const Plan& p, const float* image, const float* imageT, float* o
 {
     dim3 block(kDdFwdBlock), grid((p.npix + kDdFwdBlock - 1) / kDdFwdBlock, p.nang, nslices);
     dd_fwd_kernel<ARITH, EPI, ONES><<<grid, block, 0, st>>>(
-        (const DdView*)p.d_geom, p.nang, p.npix, image, imageT, out,
+        (const DdView*)p.d_geom, p.d_lstar, p.nang, p.npix, image, imageT, out,
         (int64_t)p.npix * p.npix, (int64_t)p.nang * p.npix, accumulate, e);
 }
 
@@ -71,7 +85,8 @@ void launch_back(const Plan& p, const float* sino, float* out, int nslices, int 
     dim3 block(256),
         grid((p.npix + kBackTile - 1) / kBackTile, (p.npix + kBackTile - 1) / kBackTile, nslices);
     dd_back_kernel<ARITH, EPI, ONES><<<grid, block, 0, st>>>(
-        (const DdView*)p.d_geom, p.nang, p.npix, sino, out, (int64_t)p.nang * p.npix,
+        (const DdView*)p.d_geom, p.d_lstar, p.nang, p.npix, sino, out,
+        (int64_t)p.nang * p.npix,
         (int64_t)p.npix * p.npix, accumulate, e);
 }
 
@@ -180,6 +195,23 @@ int dd_back(const Plan& p, const float* sino, float* image, int nslices, int64_t
 {
     EpiArgs e{};
     return dd_back_epi(p, EPI_STORE, 0, sino, image, nslices, accumulate, e, st, nullptr);
+}
+
+int dd_build_line_ends(Plan& p)
+{
+    const size_t n = (size_t)p.nang * p.npix;
+    int rc = check_hip(hipMalloc(&p.d_lstar, n * sizeof(float)), "plan_create: hipMalloc");
+    if (rc) return rc;
+    const unsigned blocks = (unsigned)((n + 255) / 256);
+    if (p.arith == ITOMO_ARITH_FAST)
+        line_end_kernel<ITOMO_ARITH_FAST><<<blocks, 256, 0, 0>>>(
+            (const DdView*)p.d_geom, p.nang, p.npix, p.d_lstar);
+    else
+        line_end_kernel<ITOMO_ARITH_IEEE><<<blocks, 256, 0, 0>>>(
+            (const DdView*)p.d_geom, p.nang, p.npix, p.d_lstar);
+    rc = check_hip(hipGetLastError(), "plan_create: line-end table");
+    if (rc) return rc;
+    return check_hip(hipStreamSynchronize(0), "plan_create: line-end table");
 }
 
 }  // namespace itomo

@@ -36,41 +36,39 @@ __device__ __forceinline__ float accum(float w, float v, float acc)
     return __fadd_rn(acc, __fmul_rn(w, v));
 }
 
-// Accumulate the pixel with boundaries [lo, hi] against all bins it overlaps, in
-// increasing bin order.  E is the dropped right end of this line.
+// Start L* of the segment a sweep line drops (radon_dd.c:176-177).  The reference
+// merges the pixel keys P_0..P_N with the bin keys D_0..D_N (on equal keys the pixel
+// key first) and stops at the first key with index N, before that segment is used;
+// L* is the key merged just before it:
+//   P_N <= D_N (stop at P_N):  L* = max(P_{N-1}, largest D_j <  P_N)
+//   D_N <  P_N (stop at D_N):  L* = max(D_{N-1}, largest P_k <= D_N)
+// Every other segment ends at or before L*, so clipping each segment's right end to L*
+// removes exactly the dropped one -- also when P_k == D_N for some k < N, where the
+// dropped segment has zero length and the one ending at P_k is kept.
 template <int ARITH>
-__device__ __forceinline__ float pixel_gather(const DdView& g, int npix, int nh, float lo,
-                                              float hi, float E,
-                                              const float* __restrict__ srow, bool ones,
-                                              float acc)
+__device__ __forceinline__ float dd_line_end(const DdView& g, int npix, int nh, int i)
 {
-    // estimate the bin containing lo, then correct it with the exact keys
-    float fe = floorf(__fmul_rn(lo, g.est));
-    fe = fminf(fmaxf(fe, (float)(-nh - 1)), (float)(npix - nh));
-    int d = (int)fe + nh;  // in [-1, npix]
-    float Dlo = key_D<ARITH>(g, d - nh);
-    float Dhi = key_D<ARITH>(g, d + 1 - nh);
-    while (d > -1 && lo < Dlo) {
-        --d;
-        Dhi = Dlo;
-        Dlo = key_D<ARITH>(g, d - nh);
-    }
-    while (d < npix && lo >= Dhi) {
-        ++d;
-        Dlo = Dhi;
-        Dhi = key_D<ARITH>(g, d + 1 - nh);
-    }
-    while (d < npix && Dlo < hi) {
-        if (d >= 0) {
-            const float R = fminf(hi, Dhi), L = fmaxf(lo, Dlo);
-            if (R > L && R != E)
-                acc = accum<ARITH>(__fsub_rn(R, L), ones ? 1.0f : srow[d], acc);
+    const float K = key_K<ARITH>(g, line_coord(i, nh));
+    const float PN = __fsub_rn((float)(npix - nh), K);
+    const float DN = key_D<ARITH>(g, npix - nh);
+    const float ninf = __int_as_float(0xff800000);
+    // binary search for the last index in [0, npix) whose key precedes the stop key;
+    // both key sequences are non-decreasing in their index
+    int lo = -1, hi = npix;
+    if (PN <= DN) {
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (key_D<ARITH>(g, mid - nh) < PN) lo = mid; else hi = mid;
         }
-        ++d;
-        Dlo = Dhi;
-        Dhi = key_D<ARITH>(g, d + 1 - nh);
+        const float Dj = lo >= 0 ? key_D<ARITH>(g, lo - nh) : ninf;
+        return fmaxf(__fsub_rn((float)(npix - 1 - nh), K), Dj);
     }
-    return acc;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (__fsub_rn((float)(mid - nh), K) <= DN) lo = mid; else hi = mid;
+    }
+    const float Pk = lo >= 0 ? __fsub_rn((float)(lo - nh), K) : ninf;
+    return fmaxf(key_D<ARITH>(g, npix - 1 - nh), Pk);
 }
 
 }  // namespace
@@ -124,9 +122,9 @@ __device__ __forceinline__ void staged_range(const DdView& g, int npix, int nh, 
 }
 
 // One line of 4 pixels with along-line indices p0..p0+3.  S points at LDS such that
-// S[d] is bin d of this view (zero outside [0, npix)).
+// S[d] is bin d of this view (zero outside [0, npix)); Ls is the line's L*.
 template <int ARITH>
-__device__ __forceinline__ void walk4(const DdView& g, int npix, int nh, float K, float E,
+__device__ __forceinline__ void walk4(const DdView& g, int npix, int nh, float K, float Ls,
                                       int p0, const float* S, float& a0, float& a1,
                                       float& a2, float& a3)
 {
@@ -152,16 +150,17 @@ __device__ __forceinline__ void walk4(const DdView& g, int npix, int nh, float K
     {                                                                                \
         x = __fadd_rn(x, 1.0f);                                                      \
         const float hi = __fsub_rn(x, K);                                            \
+        const float hc = fminf(hi, Ls);                                              \
         {                                                                            \
-            const float R = fminf(hi, Dhi), L = fmaxf(lo, Dlo);                      \
-            if (R > L && R != E) acc = accum<ARITH>(__fsub_rn(R, L), S[d], acc);     \
+            const float R = fminf(hc, Dhi), L = fmaxf(lo, Dlo);                      \
+            if (R > L) acc = accum<ARITH>(__fsub_rn(R, L), S[d], acc);               \
         }                                                                            \
         while (d < npix && Dhi < hi) {                                               \
             ++d;                                                                     \
             Dlo = Dhi;                                                               \
             Dhi = key_D<ARITH>(g, d + 1 - nh);                                       \
-            const float R = fminf(hi, Dhi);                                          \
-            if (R > Dlo && R != E) acc = accum<ARITH>(__fsub_rn(R, Dlo), S[d], acc); \
+            const float R = fminf(hc, Dhi);                                          \
+            if (R > Dlo) acc = accum<ARITH>(__fsub_rn(R, Dlo), S[d], acc);           \
         }                                                                            \
         lo = hi;                                                                     \
     }
@@ -175,7 +174,8 @@ __device__ __forceinline__ void walk4(const DdView& g, int npix, int nh, float K
 // One 64x64 tile (tile column bx, tile row by) of slice z; blk/nbx index its FP64
 // partials.  Shared by the per-plan kernel and the persistent batch kernel.
 template <int ARITH, int EPI, bool ONES>
-__device__ __forceinline__ void dd_back_tile(const DdView* __restrict__ geom, int nang, int npix,
+__device__ __forceinline__ void dd_back_tile(const DdView* __restrict__ geom,
+                                             const float* __restrict__ lstar, int nang, int npix,
                                              const float* __restrict__ sino,
                                              float* __restrict__ out, int64_t sino_stride,
                                              int64_t img_stride, int accumulate,
@@ -184,6 +184,7 @@ __device__ __forceinline__ void dd_back_tile(const DdView* __restrict__ geom, in
     __shared__ float sS[kBackViews][kBackFoot];
     __shared__ int sLo[kBackViews];
     __shared__ int sCnt[kBackViews];
+    __shared__ float sLs[kBackViews][kBackTile];   // L* of the tile's lines per view
 
     const int tid = threadIdx.x;
     const int tx = tid & 15, ty = tid >> 4;
@@ -192,7 +193,6 @@ __device__ __forceinline__ void dd_back_tile(const DdView* __restrict__ geom, in
     sino += (int64_t)z * sino_stride;
     out += (int64_t)z * img_stride;
     const int nh = (int)(npix * 0.5);
-    const float xN = (float)(npix - nh);
 
     float acc[4][4];
 #pragma unroll
@@ -223,27 +223,28 @@ __device__ __forceinline__ void dd_back_tile(const DdView* __restrict__ geom, in
                 val = ONES ? 1.0f : sino[(int64_t)geom[v0 + v].row * npix + d];
             sS[v][j] = val;
         }
+        for (int k = tid; k < nv * kBackTile; k += 256) {
+            const int v = k / kBackTile, l = k - v * kBackTile;
+            const int line = (geom[v0 + v].kind == DD_SWEEP_COLS ? q0 : r0) + l;
+            sLs[v][l] = line < npix ? lstar[(int64_t)(v0 + v) * npix + line] : 0.0f;
+        }
         __syncthreads();
         for (int v = 0; v < nv; ++v) {
             const DdView g = geom[v0 + v];
             const float* S = &sS[v][0] - sLo[v];
             if (g.kind == DD_SWEEP_ROWS) {
-                const float DN = key_D<ARITH>(g, npix - nh);
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
                     const float K = key_K<ARITH>(g, line_coord(rr + i, nh));
-                    const float E = fminf(__fsub_rn(xN, K), DN);
-                    walk4<ARITH>(g, npix, nh, K, E, cc, S, acc[i][0], acc[i][1], acc[i][2],
-                                 acc[i][3]);
+                    walk4<ARITH>(g, npix, nh, K, sLs[v][4 * ty + i], cc, S, acc[i][0],
+                                 acc[i][1], acc[i][2], acc[i][3]);
                 }
             } else if (g.kind == DD_SWEEP_COLS) {
-                const float DN = key_D<ARITH>(g, npix - nh);
 #pragma unroll
                 for (int j = 0; j < 4; ++j) {
                     const float K = key_K<ARITH>(g, line_coord(cc + j, nh));
-                    const float E = fminf(__fsub_rn(xN, K), DN);
-                    walk4<ARITH>(g, npix, nh, K, E, rr, S, acc[0][j], acc[1][j], acc[2][j],
-                                 acc[3][j]);
+                    walk4<ARITH>(g, npix, nh, K, sLs[v][4 * tx + j], rr, S, acc[0][j],
+                                 acc[1][j], acc[2][j], acc[3][j]);
                 }
             } else if (g.kind == DD_ROWSUM) {
 #pragma unroll
@@ -302,17 +303,17 @@ __device__ __forceinline__ float load_px(__amdgpu_buffer_rsrc_t rs, int idx)
 }
 
 // One candidate pixel of a bin [Dlo, Dhi] on a line with offset K: the pixel spans
-// [Plo, x+1-K].  Only the first candidate can start left of Dlo.  Advances x and Plo.
+// [Plo, x+1-K].  Only the first candidate can start left of Dlo.  Dhc is the bin's right
+// end clipped to the line's L*.  Advances x and Plo.
 template <int ARITH>
 __device__ __forceinline__ float cand_step(bool first, bool in_range, float& x, float& Plo,
-                                           float K, float Dlo, float Dhi, float E, float val,
-                                           float acc)
+                                           float K, float Dlo, float Dhc, float val, float acc)
 {
     x = __fadd_rn(x, 1.0f);
     const float Phi = __fsub_rn(x, K);
-    const float R = (Phi < Dhi) ? Phi : Dhi;
+    const float R = (Phi < Dhc) ? Phi : Dhc;
     const float L = first ? ((Plo > Dlo) ? Plo : Dlo) : Plo;
-    const bool ok = in_range && R > L && R != E;
+    const bool ok = in_range && R > L;
     const float nacc = accum<ARITH>(__fsub_rn(R, L), val, acc);
     Plo = Phi;
     return ok ? nacc : acc;
@@ -324,7 +325,8 @@ __device__ __forceinline__ float cand_step(bool first, bool in_range, float& x, 
 // hold consecutive bins, read consecutive addresses.
 // Bins [bx*kDdFwdBlock, +kDdFwdBlock) of view `view`, slice z; blk indexes the partials.
 template <int ARITH, int EPI, bool ONES>
-__device__ __forceinline__ void dd_fwd_block(const DdView* __restrict__ geom, int nang,
+__device__ __forceinline__ void dd_fwd_block(const DdView* __restrict__ geom,
+                                             const float* __restrict__ lstar, int nang,
                                              int npix, const float* __restrict__ image,
                                              const float* __restrict__ imageT,
                                              float* __restrict__ out, int64_t img_stride,
@@ -357,7 +359,7 @@ __device__ __forceinline__ void dd_fwd_block(const DdView* __restrict__ geom, in
                 image_rsrc((g.kind == DD_SWEEP_ROWS) ? image : imageT, npix);
             const float Dlo = key_D<ARITH>(g, d - nh);
             const float Dhi = key_D<ARITH>(g, d + 1 - nh);
-            const float DN = key_D<ARITH>(g, npix - nh);
+            const float* ls = lstar + (int64_t)view * npix;
             const float x0 = (float)(-nh), xN = (float)(npix - nh);
             // lines whose pixel span [x0 - K, xN - K] can meet [Dlo, Dhi]:
             // x0 - Dhi < K_i < xN - Dlo with K_i ~ (i - nh + 0.5) * t; two lines of margin,
@@ -374,7 +376,7 @@ __device__ __forceinline__ void dd_fwd_block(const DdView* __restrict__ geom, in
                 const float K = key_K<ARITH>(g, line_coord(i, nh));
                 const float P0 = __fsub_rn(x0, K), PN = __fsub_rn(xN, K);
                 if (!(PN > Dlo && P0 < Dhi)) continue;
-                const float E = fminf(PN, DN);
+                const float Dhc = fminf(Dhi, ls[i]);
                 // the pixel whose span holds Dlo: estimate, then one exact correction
                 float fe = floorf(__fadd_rn(Dlo, K));
                 fe = fminf(fmaxf(fe, (float)(-nh - 1)), (float)(npix - nh));
@@ -397,7 +399,7 @@ __device__ __forceinline__ void dd_fwd_block(const DdView* __restrict__ geom, in
                     for (int k = 0; k < 3; ++k) v[k] = ONES ? 1.0f : load_px(rs, lbase + q + k);
 #pragma unroll
                     for (int k = 0; k < 3; ++k)
-                        acc = cand_step<ARITH>(k == 0, q + k < npix, x, Plo, K, Dlo, Dhi, E, v[k], acc);
+                        acc = cand_step<ARITH>(k == 0, q + k < npix, x, Plo, K, Dlo, Dhc, v[k], acc);
                 } else {
                     for (int k0 = 0; k0 < nc; k0 += 4) {
                         float v[4];
@@ -407,7 +409,7 @@ __device__ __forceinline__ void dd_fwd_block(const DdView* __restrict__ geom, in
 #pragma unroll
                         for (int k = 0; k < 4; ++k)
                             acc = cand_step<ARITH>(k0 + k == 0, (k0 + k < nc) && (q + k0 + k < npix),
-                                                   x, Plo, K, Dlo, Dhi, E, v[k], acc);
+                                                   x, Plo, K, Dlo, Dhc, v[k], acc);
                     }
                 }
             }

@@ -57,6 +57,7 @@ struct Plan {
     int arith;       // ITOMO_ARITH_*
     void* d_geom;    // device array of DdView / RdView
     void* h_geom;    // host copy
+    float* d_lstar;  // distance-driven: [view][line] start of the dropped segment
     // Device scratch owned by the plan (FP64 partial sums, ray-driven back-projection
     // image).  Grown on first use; a plan must not be used from two streams at once.
     mutable void* scratch;
@@ -103,6 +104,8 @@ int set_error(int status, const char* fmt, ...);
 int check_hip(hipError_t e, const char* what);
 
 // launchers (dd.hip / rd.hip / iter.hip)
+// Tabulates Plan::d_lstar (allocates it; synchronous).
+int dd_build_line_ends(Plan& p);
 int dd_forward(const Plan& p, const float* image, float* sino, int nslices,
                int64_t img_stride, int64_t sino_stride, int accumulate, hipStream_t st);
 int dd_back(const Plan& p, const float* sino, float* image, int nslices,

@@ -215,7 +215,9 @@ int itomo_plan_create(itomo_plan** out, int projector, int npix, const float* an
     if (st == ITOMO_OK)
         st = check_hip(hipMemcpy(p->d_geom, p->h_geom, bytes, hipMemcpyHostToDevice),
                        "plan_create: hipMemcpy");
+    if (st == ITOMO_OK && projector == ITOMO_PROJ_DD) st = dd_build_line_ends(*p);
     if (st != ITOMO_OK) {
+        if (p->d_lstar) (void)hipFree(p->d_lstar);
         if (p->d_geom) (void)hipFree(p->d_geom);
         std::free(p->h_geom);
         delete p;
@@ -230,6 +232,10 @@ int itomo_plan_destroy(itomo_plan* p)
     if (!p) return ITOMO_OK;
     int st = ITOMO_OK;
     if (p->d_geom) st = check_hip(hipFree(p->d_geom), "plan_destroy: hipFree");
+    if (p->d_lstar) {
+        int st2 = check_hip(hipFree(p->d_lstar), "plan_destroy: hipFree");
+        if (!st) st = st2;
+    }
     if (p->scratch) {
         int st2 = check_hip(hipFree(p->scratch), "plan_destroy: hipFree scratch");
         if (!st) st = st2;

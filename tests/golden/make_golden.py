@@ -56,6 +56,26 @@ def main():
         for flags in ("fast", "strict"):
             data[key + "/fwd_" + flags] = ref.forward(proj, img, ang, flags)
             data[key + "/back_" + flags] = ref.back(proj, sino, ang, flags)
+    # Distance-driven lines where a pixel boundary P_k (k < N) equals the last detector
+    # boundary D_N exactly: the reference keeps the segment ending at P_k and drops only
+    # the zero-length one after it.  Views 3 and 15 of the 24-view, 1024-pixel geometry
+    # (after fftshift) each hold such a line (row 623 / column 400).  Image and sinogram
+    # are nonzero only near the far edge, where those segments lie.
+    cang = np.fft.fftshift(facade_angles(angles_deg(24)))[[3, 15]]
+    ang = np.fft.ifftshift(cang)      # the wrapper's fftshift restores [view 3, view 15]
+    rng = np.random.default_rng(300)
+    img = np.zeros((1024, 1024), np.float32)
+    img[1016:, :] = rng.random((8, 1024))
+    img[:, 1016:] = rng.random((1024, 8))
+    sino = np.zeros((2, 1024), np.float32)
+    sino[:, 1016:] = rng.random((2, 8)) + 0.5
+    key = "dd_tie_1024_2"
+    data[key + "/angles"] = ang
+    data[key + "/image"] = img
+    data[key + "/sino"] = sino
+    for flags in ("fast", "strict"):
+        data[key + "/fwd_" + flags] = ref.forward("dd", img, ang, flags)
+        data[key + "/back_" + flags] = ref.back("dd", sino, ang, flags)
     np.savez_compressed(OUT, **data)
     print("wrote %s (%d arrays, %.1f MB)" % (OUT, len(data), os.path.getsize(OUT) / 1e6))
 

@@ -172,3 +172,32 @@ def test_dd_sinogram_row_order_matches_wrapper():
     s = O.dd_forward(img, ang, O.F64)
     np.testing.assert_allclose(s[V - 1 - V // 2], img.sum(axis=1), rtol=1e-12)
     np.testing.assert_allclose(s[V - 1], img.sum(axis=0), rtol=1e-12)
+
+
+def test_tie_fixture_holds_a_dropped_segment_tie():
+    """The dd_tie fixture exists for lines where a pixel boundary P_k (k < N) equals the
+    last detector boundary D_N: the reference keeps the segment ending at P_k and drops
+    the zero-length one after it.  Pin that its second view really has such a line
+    (column 400, P_1020 == D_1024 < P_1024 in the -ffast-math key arithmetic) and that
+    the oracle credits pixel 1019 of that column with the full segment."""
+    import ctypes
+    z = np.load(GOLDEN)
+    key = "dd_tie_1024_2"
+    theta = np.fft.fftshift(z[key + "/angles"])[1]
+    m = ctypes.CDLL("libm.so.6")
+    m.sincosf.argtypes = [ctypes.c_float, ctypes.POINTER(ctypes.c_float),
+                          ctypes.POINTER(ctypes.c_float)]
+    s_, c_ = ctypes.c_float(), ctypes.c_float()
+    m.sincosf(ctypes.c_float(theta), ctypes.byref(s_), ctypes.byref(c_))
+    f32 = np.float32
+    rc = f32(1) / f32(c_.value)
+    t = f32(s_.value) * rc
+    K = f32(-111.5) * t                       # column 400: x = (400 - 512) + 0.5
+    P = (np.arange(1025) - 512).astype(f32) - K
+    DN = f32(512) * rc
+    assert P[1020] == DN and DN < P[1024]
+    row = z[key + "/sino"][0]                 # view v = 1 lands in row nang-1-v = 0
+    one = np.zeros_like(z[key + "/sino"])
+    one[0] = row
+    got = O.dd_back(one[:1], np.array([theta], np.float32), O.FAST)
+    assert got[1019, 400] == f32(P[1020] - P[1019]) * row[1023] != 0

@@ -168,10 +168,21 @@ def test_full_size_forward_views_bitwise(full_plan):
         np.testing.assert_array_equal(sino[V_FULL - 1 - v], ref[0], err_msg="view %d" % v)
 
 
+def test_full_size_forward_tie_views_bitwise(full_plan):
+    # views 16, 31, 34 hold lines where a pixel boundary equals the last detector
+    # boundary exactly; a full-field random image puts weight on those edge segments
+    p, ang = full_plan
+    img = np.random.default_rng(5).random((N_FULL, N_FULL)).astype(np.float32)
+    sino = p.forward(gpu(img)).cpu().numpy()
+    for v in (16, 31, 34):
+        ref = O.dd_forward(img, np.array([ang[v]], np.float32), O.FAST)
+        np.testing.assert_array_equal(sino[V_FULL - 1 - v], ref[0], err_msg="view %d" % v)
+
+
 def test_full_size_back_single_views_bitwise(full_plan):
     p, ang = full_plan
     rng = np.random.default_rng(3)
-    for v in (3, 899, 1200, 1799):
+    for v in (3, 16, 31, 899, 1200, 1799):
         sino = np.zeros((V_FULL, N_FULL), np.float32)
         sino[V_FULL - 1 - v] = rng.random(N_FULL).astype(np.float32)
         got = p.back(gpu(sino)).cpu().numpy()
