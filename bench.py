@@ -84,6 +84,7 @@ def main():
 
     import numpy as np
     import torch
+    from iterative_tomographic_reconstruction_algorithms_amd import autotune
     from iterative_tomographic_reconstruction_algorithms_amd.core import Plan
     from iterative_tomographic_reconstruction_algorithms_amd.phantom import angles_deg, ellipses
 
@@ -93,6 +94,8 @@ def main():
     facade = (angles_deg(V) * np.pi / 180).astype(np.float32)
     c_angles = np.fft.fftshift(facade)
     plan = Plan("dd", N, c_angles)
+    # launch shapes from the autotuner (timed here, not read from a cache)
+    shapes = autotune.tune(plan, nslices=1, reps=2, use_cache=False)
     x_true = torch.from_numpy(img).to(dev)
     b = plan.forward(x_true)
     R, C = plan.sirt_weights()
@@ -124,9 +127,11 @@ def main():
     resid_rel = float(stats[0].sqrt()) / bnorm
 
     if back_ms >= fwd_ms:
-        dom, dom_ms, dom_bytes = "dd_back_kernel<FAST,SIRT_BACK>", back_ms, alg_bytes_back(N, V)
+        dom, dom_ms, dom_bytes = "dd_back_kernel<FAST,SIRT_BACK,%d,%d>" % tuple(
+            [(64, 4), (32, 2), (32, 4)][shapes["back"]]), back_ms, alg_bytes_back(N, V)
     else:
-        dom, dom_ms, dom_bytes = "dd_fwd_kernel<FAST,SIRT_FWD>", fwd_ms, alg_bytes_forward(N, V)
+        dom, dom_ms, dom_bytes = "dd_fwd_kernel<FAST,SIRT_FWD,%d>" % (
+            [256, 128, 64][shapes["fwd"]]), fwd_ms, alg_bytes_forward(N, V)
     achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
     line = {
         "metric": "SIRT Gupdates/s & s-to-residual 1e-3 (plain vs warm-start) @2048²×1800 views",
@@ -143,7 +148,9 @@ def main():
         "data": "synthetic",
         "config": {"workload": "plain SIRT, distance-driven pair, 2048x2048 image, 1800 views "
                                "over [0,180), 2048 bins, 40-ellipse phantom (seed 2), noiseless, "
-                               "x0=0, non-negativity on", "npix": N, "nang": V, "slices": 1},
+                               "x0=0, non-negativity on", "npix": N, "nang": V, "slices": 1,
+                   "shapes": {"fwd": shapes["fwd"], "back": shapes["back"],
+                              "fwd_ms": shapes["fwd_ms"], "back_ms": shapes["back_ms"]}},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": None},

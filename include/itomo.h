@@ -75,6 +75,19 @@ int itomo_plan_destroy(itomo_plan *plan);
 /* Host copy of the per-view sin/cos the plan uses (for checking). */
 int itomo_plan_trig(const itomo_plan *plan, float *s_host, float *c_host);
 
+/* Launch shapes of the distance-driven kernels (no counterpart in the reference, whose
+ * C has no launch shapes; used by the autotuner).  Forward variant f in
+ * [0, ITOMO_FWD_VARIANTS): 256 / 128 / 64 bins per block.  Back variant b in
+ * [0, ITOMO_BACK_VARIANTS): 64x64 tile with 4x4 pixels per thread / 32x32 with 2x2 /
+ * 32x32 with 4x4.  Every shape produces bit-identical projections and iterates; FP64
+ * monitor sums follow the block partition and may differ in the last bits.
+ * Ray-driven plans accept only (0, 0).  Not to be called while work using the plan is
+ * queued on another stream. */
+#define ITOMO_FWD_VARIANTS 3
+#define ITOMO_BACK_VARIANTS 3
+int itomo_plan_set_shapes(itomo_plan *plan, int fwd_variant, int back_variant);
+int itomo_plan_get_shapes(const itomo_plan *plan, int *fwd_variant, int *back_variant);
+
 /* Forward projection A (class_projectors_radon.py:60-68): sino = A image.
  * accumulate = 0 overwrites sino, 1 adds to it (the reference C always adds into the
  * zeroed array its wrapper allocates).  Deterministic. */

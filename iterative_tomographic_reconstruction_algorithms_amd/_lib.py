@@ -34,6 +34,8 @@ _SIGS = {
     "itomo_plan_create": ([ctypes.POINTER(_vp), _i, _i, _vp, _i, _i], _i),
     "itomo_plan_destroy": ([_vp], _i),
     "itomo_plan_trig": ([_vp, _vp, _vp], _i),
+    "itomo_plan_set_shapes": ([_vp, _i, _i], _i),
+    "itomo_plan_get_shapes": ([_vp, _vp, _vp], _i),
     "itomo_forward": ([_vp, _vp, _vp, _i, _i, _vp], _i),
     "itomo_back": ([_vp, _vp, _vp, _i, _i, _vp], _i),
     "itomo_sirt_weights": ([_vp, _vp, _vp, _vp], _i),
@@ -67,6 +69,8 @@ _SIGS = {
     "itomo_filter_rows": ([_vp, _vp, _i, _i, _vp, _vp], _i),
 }
 
+FWD_VARIANTS = 3      # ITOMO_FWD_VARIANTS
+BACK_VARIANTS = 3     # ITOMO_BACK_VARIANTS
 FILTERS = {"ramp": 0, "shepp-logan": 1, "cosine": 2, "hamming": 3, "hanning": 4, "none": 5}
 
 VEC_WORK_PER_SLICE = 771

@@ -102,6 +102,21 @@ class Plan:
                                         c.ctypes.data_as(ctypes.c_void_p)), "plan_trig")
         return s, c
 
+    # -- launch shapes --------------------------------------------------------------
+    @property
+    def shapes(self):
+        """(forward variant, back variant) of the distance-driven launches."""
+        f, b = ctypes.c_int(), ctypes.c_int()
+        L.check(L.lib().itomo_plan_get_shapes(self._h, ctypes.byref(f), ctypes.byref(b)),
+                "plan_get_shapes")
+        return f.value, b.value
+
+    def set_shapes(self, fwd_variant, back_variant):
+        """Select launch shapes (see include/itomo.h); results are bit-identical for all."""
+        torch.cuda.current_stream().synchronize()
+        L.check(L.lib().itomo_plan_set_shapes(self._h, int(fwd_variant), int(back_variant)),
+                "plan_set_shapes")
+
     # -- projectors -----------------------------------------------------------------
     def forward(self, image, out=None, accumulate=False):
         """A image: (..., N, N) -> (..., V, N)."""

@@ -96,7 +96,7 @@ batch_fwd_kernel(const BatchMember* __restrict__ mem, const BatchItem* __restric
         e.partials = m.pf;
         e.blocks_per_slice = m.bps_f;
         const int nbx = (m.npix + kDdFwdBlock - 1) / kDdFwdBlock;
-        dd_fwd_block<ITOMO_ARITH_FAST, EPI_SIRT_FWD, false>(
+        dd_fwd_block<ITOMO_ARITH_FAST, EPI_SIRT_FWD, false, kDdFwdBlock>(
             m.geom, m.lstar, m.nang, m.npix, m.x, m.xT, m.work, (int64_t)m.npix * m.npix,
             (int64_t)m.nang * m.npix, 0, e, w.a, w.b, w.z, w.b * nbx + w.a);
     }
@@ -117,7 +117,7 @@ batch_back_kernel(const BatchMember* __restrict__ mem, const BatchItem* __restri
         e.partials = m.pb;
         e.blocks_per_slice = m.bps_b;
         const int nbx = (m.npix + kDdBackTile - 1) / kDdBackTile;
-        dd_back_tile<ITOMO_ARITH_FAST, EPI_SIRT_BACK, false>(
+        dd_back_tile<ITOMO_ARITH_FAST, EPI_SIRT_BACK, false, kDdBackTile, 4>(
             m.geom, m.lstar, m.nang, m.npix, m.work, m.x, (int64_t)m.nang * m.npix,
             (int64_t)m.npix * m.npix, 0, e, w.a, w.b, w.z, w.b * nbx + w.a);
     }
@@ -210,8 +210,9 @@ int itomo_batch_create(itomo_batch** out, int nmembers, const itomo_plan* const*
         bm.R = R[m];
         bm.C = C[m];
         bm.work = work[m];
-        bm.bps_f = (int)fwd_blocks_per_slice(*p);
-        bm.bps_b = (int)back_blocks_per_slice(*p);
+        // the batch launches the default shapes whatever the plan's own choice
+        bm.bps_f = (int)dd_fwd_blocks(p->npix, p->nang, kDdFwdBlock);
+        bm.bps_b = (int)dd_back_blocks(p->npix, kDdBackTile);
         B->h_members.push_back(bm);
         B->plans.push_back(p);
         off_xT[m] = scratch_bytes;

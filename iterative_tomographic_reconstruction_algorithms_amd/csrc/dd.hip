@@ -23,25 +23,25 @@
 
 namespace itomo {
 
-template <int ARITH, int EPI, bool ONES>
-__global__ void __launch_bounds__(256)
+template <int ARITH, int EPI, bool ONES, int T, int P>
+__global__ void __launch_bounds__((T / P) * (T / P))
 dd_back_kernel(const DdView* __restrict__ geom, const float* __restrict__ lstar, int nang,
                int npix, const float* __restrict__ sino, float* __restrict__ out,
                int64_t sino_stride, int64_t img_stride, int accumulate, EpiArgs e)
 {
-    dd_back_tile<ARITH, EPI, ONES>(geom, lstar, nang, npix, sino, out, sino_stride, img_stride,
+    dd_back_tile<ARITH, EPI, ONES, T, P>(geom, lstar, nang, npix, sino, out, sino_stride, img_stride,
                                    accumulate, e, blockIdx.x, blockIdx.y, blockIdx.z,
                                    blockIdx.y * gridDim.x + blockIdx.x);
 }
 
-template <int ARITH, int EPI, bool ONES>
-__global__ void __launch_bounds__(kDdFwdBlock)
+template <int ARITH, int EPI, bool ONES, int B>
+__global__ void __launch_bounds__(B)
 dd_fwd_kernel(const DdView* __restrict__ geom, const float* __restrict__ lstar, int nang,
               int npix, const float* __restrict__ image, const float* __restrict__ imageT,
               float* __restrict__ out, int64_t img_stride, int64_t sino_stride,
               int accumulate, EpiArgs e)
 {
-    dd_fwd_block<ARITH, EPI, ONES>(geom, lstar, nang, npix, image, imageT, out, img_stride,
+    dd_fwd_block<ARITH, EPI, ONES, B>(geom, lstar, nang, npix, image, imageT, out, img_stride,
                                    sino_stride, accumulate, e, blockIdx.x, blockIdx.y,
                                    blockIdx.z, blockIdx.y * gridDim.x + blockIdx.x);
 }
@@ -68,26 +68,50 @@ line_end_kernel(const DdView* __restrict__ geom, int nang, int npix, float* __re
 
 namespace {
 
+template <int ARITH, int EPI, bool ONES, int B>
+void launch_fwd_b(const Plan& p, const float* image, const float* imageT, float* out,
+                  int nslices, int accumulate, const EpiArgs& e, hipStream_t st)
+{
+    dim3 block(B), grid((p.npix + B - 1) / B, p.nang, nslices);
+    dd_fwd_kernel<ARITH, EPI, ONES, B><<<grid, block, 0, st>>>(
+        (const DdView*)p.d_geom, p.d_lstar, p.nang, p.npix, image, imageT, out,
+        (int64_t)p.npix * p.npix, (int64_t)p.nang * p.npix, accumulate, e);
+}
+
 template <int ARITH, int EPI, bool ONES>
 void launch_fwd(const Plan& p, const float* image, const float* imageT, float* out,
                 int nslices, int accumulate, const EpiArgs& e, hipStream_t st)
 {
-    dim3 block(kDdFwdBlock), grid((p.npix + kDdFwdBlock - 1) / kDdFwdBlock, p.nang, nslices);
-    dd_fwd_kernel<ARITH, EPI, ONES><<<grid, block, 0, st>>>(
-        (const DdView*)p.d_geom, p.d_lstar, p.nang, p.npix, image, imageT, out,
-        (int64_t)p.npix * p.npix, (int64_t)p.nang * p.npix, accumulate, e);
+    static_assert(kFwdBins[0] == 256 && kFwdBins[1] == 128 && kFwdBins[2] == 64, "variants");
+    switch (p.fwd_var) {
+    case 1: launch_fwd_b<ARITH, EPI, ONES, 128>(p, image, imageT, out, nslices, accumulate, e, st); break;
+    case 2: launch_fwd_b<ARITH, EPI, ONES, 64>(p, image, imageT, out, nslices, accumulate, e, st); break;
+    default: launch_fwd_b<ARITH, EPI, ONES, 256>(p, image, imageT, out, nslices, accumulate, e, st); break;
+    }
+}
+
+template <int ARITH, int EPI, bool ONES, int T, int P>
+void launch_back_tp(const Plan& p, const float* sino, float* out, int nslices, int accumulate,
+                    const EpiArgs& e, hipStream_t st)
+{
+    dim3 block((T / P) * (T / P)), grid((p.npix + T - 1) / T, (p.npix + T - 1) / T, nslices);
+    dd_back_kernel<ARITH, EPI, ONES, T, P><<<grid, block, 0, st>>>(
+        (const DdView*)p.d_geom, p.d_lstar, p.nang, p.npix, sino, out,
+        (int64_t)p.nang * p.npix, (int64_t)p.npix * p.npix, accumulate, e);
 }
 
 template <int ARITH, int EPI, bool ONES>
 void launch_back(const Plan& p, const float* sino, float* out, int nslices, int accumulate,
                  const EpiArgs& e, hipStream_t st)
 {
-    dim3 block(256),
-        grid((p.npix + kBackTile - 1) / kBackTile, (p.npix + kBackTile - 1) / kBackTile, nslices);
-    dd_back_kernel<ARITH, EPI, ONES><<<grid, block, 0, st>>>(
-        (const DdView*)p.d_geom, p.d_lstar, p.nang, p.npix, sino, out,
-        (int64_t)p.nang * p.npix,
-        (int64_t)p.npix * p.npix, accumulate, e);
+    static_assert(kBackTiles[0][0] == 64 && kBackTiles[0][1] == 4 && kBackTiles[1][0] == 32 &&
+                      kBackTiles[1][1] == 2 && kBackTiles[2][0] == 32 && kBackTiles[2][1] == 4,
+                  "variants");
+    switch (p.back_var) {
+    case 1: launch_back_tp<ARITH, EPI, ONES, 32, 2>(p, sino, out, nslices, accumulate, e, st); break;
+    case 2: launch_back_tp<ARITH, EPI, ONES, 32, 4>(p, sino, out, nslices, accumulate, e, st); break;
+    default: launch_back_tp<ARITH, EPI, ONES, 64, 4>(p, sino, out, nslices, accumulate, e, st); break;
+    }
 }
 
 template <int ARITH>

@@ -8,13 +8,18 @@ namespace itomo {
 
 namespace {
 
-// Bin boundary D for j = d - nh.
+// Bin boundary D for j = d - nh, with j given as an integer-valued float.
+template <int ARITH>
+__device__ __forceinline__ float key_Df(const DdView& g, float x)
+{
+    if (ARITH == ITOMO_ARITH_FAST) return __fmul_rn(x, g.rinv);
+    return __fdiv_rn(x, g.b);
+}
+
 template <int ARITH>
 __device__ __forceinline__ float key_D(const DdView& g, int j)
 {
-    const float x = (float)j;
-    if (ARITH == ITOMO_ARITH_FAST) return __fmul_rn(x, g.rinv);
-    return __fdiv_rn(x, g.b);
+    return key_Df<ARITH>(g, (float)j);
 }
 
 // Line offset K for line coordinate u = (i - nh) + 0.5.
@@ -75,37 +80,41 @@ __device__ __forceinline__ float dd_line_end(const DdView& g, int npix, int nh, 
 
 // ---- back projection ----------------------------------------------------------------
 //
-// Block = 64x64 pixel tile, 256 threads, each thread a 4x4 micro-tile (rows 4*ty.., columns
-// 4*tx..).  For a row sweep a thread walks its 4 rows, 4 consecutive pixels each; for a
-// column sweep its 4 columns.  Along one line the bin walk is incremental: the first
-// pixel's bin is found once, then each next pixel continues from the previous pixel's
-// last bin.  Views are processed in chunks of kBackViews: the sinogram values the tile
-// can touch in each view (at most kBackFoot bins) are staged in LDS, with zeros for
-// bins outside [0, npix), so the walk needs no bounds checks.  The accumulation order
-// per pixel (views in list order, bins increasing) is the reference's.
-constexpr int kBackTile = kDdBackTile;
+// Block = T x T pixel tile, (T/P)^2 threads, each thread a P x P micro-tile (rows
+// P*ty.., columns P*tx..).  For a row sweep a thread walks its P rows, P consecutive
+// pixels each; for a column sweep its P columns.  Along one line the bin walk is
+// incremental: the first pixel's bin is found once, then each next pixel continues from
+// the previous pixel's last bin.  Views are processed in chunks of kBackViews: the
+// sinogram values the tile can touch in each view (at most back_foot(T) bins) are staged
+// in LDS, with zeros for bins outside [0, npix), so the walk needs no bounds checks.  The
+// accumulation order per pixel (views in list order, bins increasing) is the reference's
+// for every (T, P), so all tilings give bit-identical images.
 constexpr int kBackViews = 32;
-constexpr int kBackFoot = 100;  // >= 64*(|sin|+|cos|) + 5 for every view
 
-template <int ARITH>
+// Staged bins per view: the tile's footprint spans T*(|sin|+|cos|) <= T*sqrt(2) bins,
+// plus 7 for the floors and the +-2 margins of staged_range.
+template <int T>
+constexpr int back_foot() { return (T * 1449) / 1024 + 10; }
+
+template <int ARITH, int T>
 __device__ __forceinline__ void staged_range(const DdView& g, int npix, int nh, int r0,
                                              int q0, int* b_lo, int* cnt)
 {
     int lo, hi;
     if (g.kind == DD_ROWSUM) {
         lo = r0;
-        hi = r0 + kBackTile - 1;
+        hi = r0 + T - 1;
     } else if (g.kind == DD_COLSUM) {
         lo = q0;
-        hi = q0 + kBackTile - 1;
+        hi = q0 + T - 1;
     } else if (g.kind == DD_SWEEP_ROWS || g.kind == DD_SWEEP_COLS) {
         // lines are rows (r) or columns (q); pixel boundaries along the line
         const int l0 = (g.kind == DD_SWEEP_ROWS) ? r0 : q0;
         const int p0 = (g.kind == DD_SWEEP_ROWS) ? q0 : r0;
-        const float ka = (float)(l0 - nh) + 0.5f, kb = ka + (float)(kBackTile - 1);
+        const float ka = (float)(l0 - nh) + 0.5f, kb = ka + (float)(T - 1);
         const float K1 = ka * g.t, K2 = kb * g.t;
         const float kmin = (float)(p0 - nh) - fmaxf(K1, K2);
-        const float kmax = (float)(p0 + kBackTile - nh) - fminf(K1, K2);
+        const float kmax = (float)(p0 + T - nh) - fminf(K1, K2);
         float f1 = floorf(kmin * g.est), f2 = floorf(kmax * g.est);
         f1 = fminf(fmaxf(f1, (float)(-nh - 4)), (float)(npix - nh + 4));
         f2 = fminf(fmaxf(f2, (float)(-nh - 4)), (float)(npix - nh + 4));
@@ -118,62 +127,75 @@ __device__ __forceinline__ void staged_range(const DdView& g, int npix, int nh, 
     lo = max(lo, -1);
     hi = min(hi, npix);
     *b_lo = lo;
-    *cnt = max(0, min(hi - lo + 1, kBackFoot));
+    *cnt = max(0, min(hi - lo + 1, back_foot<T>()));
 }
 
-// One line of 4 pixels with along-line indices p0..p0+3.  S points at LDS such that
-// S[d] is bin d of this view (zero outside [0, npix)); Ls is the line's L*.
-template <int ARITH>
-__device__ __forceinline__ void walk4(const DdView& g, int npix, int nh, float K, float Ls,
-                                      int p0, const float* S, float& a0, float& a1,
-                                      float& a2, float& a3)
+// One line of P pixels with along-line indices p0..p0+P-1, accumulating into
+// a[0], a[STRIDE], ..., a[(P-1)*STRIDE].  S points at LDS such that S[d] is bin d of
+// this view (zero outside [0, npix)); Ls is the line's L*.
+//
+// Bins are at least as wide as pixels (1/|sin| or 1/|cos| >= 1), so a pixel crosses at
+// most one bin boundary, except when the two widths agree to rounding: the crossing is
+// evaluated with selects, and a wave-uniform loop picks up the rare further ones.  Bins
+// are visited in increasing order either way.  Past the last bin the weights vanish by
+// the L* clip (Ls <= D[N]), so the walk needs no bound there.
+template <int ARITH, int P, int STRIDE>
+__device__ __forceinline__ void walk_line(const DdView& g, int npix, int nh, float K, float Ls,
+                                          int p0, const float* S, float* a)
 {
     float x = (float)(p0 - nh);
     float lo = __fsub_rn(x, K);
-    float fe = floorf(__fmul_rn(lo, g.est));
-    fe = fminf(fmaxf(fe, (float)(-nh - 1)), (float)(npix - nh));
-    int d = (int)fe + nh;  // in [-1, npix]
-    float Dlo = key_D<ARITH>(g, d - nh);
-    float Dhi = key_D<ARITH>(g, d + 1 - nh);
-    while (d > -1 && lo < Dlo) {
-        --d;
-        Dhi = Dlo;
-        Dlo = key_D<ARITH>(g, d - nh);
+    // (index - nh) of the bin holding lo: estimate, one exact correction, then the
+    // clamp to [-1, npix] (bins -1 and npix read staged zeros)
+    float fd = floorf(__fmul_rn(lo, g.est));
+    {
+        const float D0 = key_Df<ARITH>(g, fd);
+        const float D1 = key_Df<ARITH>(g, __fadd_rn(fd, 1.0f));
+        const float up = (lo >= D1) ? 1.0f : 0.0f;
+        const float dn = (lo < D0) ? 1.0f : 0.0f;
+        fd = __fsub_rn(__fadd_rn(fd, up), dn);
+        fd = fminf(fmaxf(fd, (float)(-nh - 1)), (float)(npix - nh));
     }
-    while (d < npix && lo >= Dhi) {
-        ++d;
-        Dlo = Dhi;
-        Dhi = key_D<ARITH>(g, d + 1 - nh);
+    int d = (int)fd + nh;
+    float Dlo = key_Df<ARITH>(g, fd);
+    float Dhi = key_Df<ARITH>(g, __fadd_rn(fd, 1.0f));
+#define ITOMO_PIX(acc)                                                                  \
+    {                                                                                   \
+        x = __fadd_rn(x, 1.0f);                                                         \
+        const float hi = __fsub_rn(x, K);                                               \
+        const float hc = fminf(hi, Ls);                                                 \
+        const float s0 = S[d], s1 = S[d + 1];                                           \
+        const float R = fminf(hc, Dhi), L = fmaxf(lo, Dlo);                             \
+        const float n0 = accum<ARITH>(__fsub_rn(R, L), s0, acc);                        \
+        acc = (R > L) ? n0 : acc;                                                       \
+        const float fd2 = __fadd_rn(fd, 1.0f);                                          \
+        const float Dhi2 = key_Df<ARITH>(g, __fadd_rn(fd2, 1.0f));                      \
+        const bool cross = Dhi < hi;                                                    \
+        const float R2 = fminf(hc, Dhi2);                                               \
+        const float n1 = accum<ARITH>(__fsub_rn(R2, Dhi), s1, acc);                     \
+        acc = (cross && R2 > Dhi) ? n1 : acc;                                           \
+        fd = cross ? fd2 : fd;                                                          \
+        d = cross ? d + 1 : d;                                                          \
+        Dlo = cross ? Dhi : Dlo;                                                        \
+        Dhi = cross ? Dhi2 : Dhi;                                                       \
+        while (d < npix && Dhi < hi) {                                                  \
+            ++d;                                                                        \
+            fd = __fadd_rn(fd, 1.0f);                                                   \
+            Dlo = Dhi;                                                                  \
+            Dhi = key_Df<ARITH>(g, __fadd_rn(fd, 1.0f));                                \
+            const float R3 = fminf(hc, Dhi);                                            \
+            if (R3 > Dlo) acc = accum<ARITH>(__fsub_rn(R3, Dlo), S[d], acc);            \
+        }                                                                               \
+        lo = hi;                                                                        \
     }
-    // invariant from here on: Dlo <= lo <= Dhi
-#define ITOMO_PIX(acc)                                                               \
-    {                                                                                \
-        x = __fadd_rn(x, 1.0f);                                                      \
-        const float hi = __fsub_rn(x, K);                                            \
-        const float hc = fminf(hi, Ls);                                              \
-        {                                                                            \
-            const float R = fminf(hc, Dhi), L = fmaxf(lo, Dlo);                      \
-            if (R > L) acc = accum<ARITH>(__fsub_rn(R, L), S[d], acc);               \
-        }                                                                            \
-        while (d < npix && Dhi < hi) {                                               \
-            ++d;                                                                     \
-            Dlo = Dhi;                                                               \
-            Dhi = key_D<ARITH>(g, d + 1 - nh);                                       \
-            const float R = fminf(hc, Dhi);                                          \
-            if (R > Dlo) acc = accum<ARITH>(__fsub_rn(R, Dlo), S[d], acc);           \
-        }                                                                            \
-        lo = hi;                                                                     \
-    }
-    ITOMO_PIX(a0)
-    ITOMO_PIX(a1)
-    ITOMO_PIX(a2)
-    ITOMO_PIX(a3)
+#pragma unroll
+    for (int k = 0; k < P; ++k) ITOMO_PIX(a[k * STRIDE])
 #undef ITOMO_PIX
 }
 
-// One 64x64 tile (tile column bx, tile row by) of slice z; blk/nbx index its FP64
+// One T x T tile (tile column bx, tile row by) of slice z; blk indexes its FP64
 // partials.  Shared by the per-plan kernel and the persistent batch kernel.
-template <int ARITH, int EPI, bool ONES>
+template <int ARITH, int EPI, bool ONES, int T, int P>
 __device__ __forceinline__ void dd_back_tile(const DdView* __restrict__ geom,
                                              const float* __restrict__ lstar, int nang, int npix,
                                              const float* __restrict__ sino,
@@ -181,24 +203,25 @@ __device__ __forceinline__ void dd_back_tile(const DdView* __restrict__ geom,
                                              int64_t img_stride, int accumulate,
                                              const EpiArgs& e, int bx, int by, int z, int blk)
 {
-    __shared__ float sS[kBackViews][kBackFoot];
+    constexpr int TP = T / P, NT = TP * TP, FOOT = back_foot<T>();
+    __shared__ float sS[kBackViews][FOOT];
     __shared__ int sLo[kBackViews];
     __shared__ int sCnt[kBackViews];
-    __shared__ float sLs[kBackViews][kBackTile];   // L* of the tile's lines per view
+    __shared__ float sLs[kBackViews][T];   // L* of the tile's lines per view
 
     const int tid = threadIdx.x;
-    const int tx = tid & 15, ty = tid >> 4;
-    const int r0 = by * kBackTile, q0 = bx * kBackTile;
-    const int rr = r0 + 4 * ty, cc = q0 + 4 * tx;  // micro-tile origin
+    const int tx = tid % TP, ty = tid / TP;
+    const int r0 = by * T, q0 = bx * T;
+    const int rr = r0 + P * ty, cc = q0 + P * tx;  // micro-tile origin
     sino += (int64_t)z * sino_stride;
     out += (int64_t)z * img_stride;
     const int nh = (int)(npix * 0.5);
 
-    float acc[4][4];
+    float acc[P][P];
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < P; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
+        for (int j = 0; j < P; ++j) {
             const int r = rr + i, q = cc + j;
             acc[i][j] = (EPI == EPI_STORE && accumulate && r < npix && q < npix)
                             ? out[(int64_t)r * npix + q]
@@ -210,21 +233,21 @@ __device__ __forceinline__ void dd_back_tile(const DdView* __restrict__ geom,
         __syncthreads();  // previous chunk fully consumed
         if (tid < nv) {
             int lo, cnt;
-            staged_range<ARITH>(geom[v0 + tid], npix, nh, r0, q0, &lo, &cnt);
+            staged_range<ARITH, T>(geom[v0 + tid], npix, nh, r0, q0, &lo, &cnt);
             sLo[tid] = lo;
             sCnt[tid] = cnt;
         }
         __syncthreads();
-        for (int k = tid; k < nv * kBackFoot; k += 256) {
-            const int v = k / kBackFoot, j = k - v * kBackFoot;
+        for (int k = tid; k < nv * FOOT; k += NT) {
+            const int v = k / FOOT, j = k - v * FOOT;
             const int d = sLo[v] + j;
             float val = 0.0f;
             if (j < sCnt[v] && d >= 0 && d < npix)
                 val = ONES ? 1.0f : sino[(int64_t)geom[v0 + v].row * npix + d];
             sS[v][j] = val;
         }
-        for (int k = tid; k < nv * kBackTile; k += 256) {
-            const int v = k / kBackTile, l = k - v * kBackTile;
+        for (int k = tid; k < nv * T; k += NT) {
+            const int v = k / T, l = k - v * T;
             const int line = (geom[v0 + v].kind == DD_SWEEP_COLS ? q0 : r0) + l;
             sLs[v][l] = line < npix ? lstar[(int64_t)(v0 + v) * npix + line] : 0.0f;
         }
@@ -234,31 +257,31 @@ __device__ __forceinline__ void dd_back_tile(const DdView* __restrict__ geom,
             const float* S = &sS[v][0] - sLo[v];
             if (g.kind == DD_SWEEP_ROWS) {
 #pragma unroll
-                for (int i = 0; i < 4; ++i) {
+                for (int i = 0; i < P; ++i) {
                     const float K = key_K<ARITH>(g, line_coord(rr + i, nh));
-                    walk4<ARITH>(g, npix, nh, K, sLs[v][4 * ty + i], cc, S, acc[i][0],
-                                 acc[i][1], acc[i][2], acc[i][3]);
+                    walk_line<ARITH, P, 1>(g, npix, nh, K, sLs[v][P * ty + i], cc, S,
+                                           &acc[i][0]);
                 }
             } else if (g.kind == DD_SWEEP_COLS) {
 #pragma unroll
-                for (int j = 0; j < 4; ++j) {
+                for (int j = 0; j < P; ++j) {
                     const float K = key_K<ARITH>(g, line_coord(cc + j, nh));
-                    walk4<ARITH>(g, npix, nh, K, sLs[v][4 * tx + j], rr, S, acc[0][j],
-                                 acc[1][j], acc[2][j], acc[3][j]);
+                    walk_line<ARITH, P, P>(g, npix, nh, K, sLs[v][P * tx + j], rr, S,
+                                           &acc[0][j]);
                 }
             } else if (g.kind == DD_ROWSUM) {
 #pragma unroll
-                for (int i = 0; i < 4; ++i) {
+                for (int i = 0; i < P; ++i) {
                     const float sv = (rr + i < npix) ? S[rr + i] : 0.0f;
 #pragma unroll
-                    for (int j = 0; j < 4; ++j) acc[i][j] = __fadd_rn(acc[i][j], sv);
+                    for (int j = 0; j < P; ++j) acc[i][j] = __fadd_rn(acc[i][j], sv);
                 }
             } else if (g.kind == DD_COLSUM) {
 #pragma unroll
-                for (int j = 0; j < 4; ++j) {
+                for (int j = 0; j < P; ++j) {
                     const float sv = (cc + j < npix) ? S[cc + j] : 0.0f;
 #pragma unroll
-                    for (int i = 0; i < 4; ++i) acc[i][j] = __fadd_rn(acc[i][j], sv);
+                    for (int i = 0; i < P; ++i) acc[i][j] = __fadd_rn(acc[i][j], sv);
                 }
             }
         }
@@ -266,9 +289,9 @@ __device__ __forceinline__ void dd_back_tile(const DdView* __restrict__ geom,
 
     double p1 = 0.0, p2 = 0.0;
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < P; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
+        for (int j = 0; j < P; ++j) {
             const int r = rr + i, q = cc + j;
             if (r < npix && q < npix) {
                 const int64_t o = (int64_t)r * npix + q;
@@ -306,25 +329,40 @@ __device__ __forceinline__ float load_px(__amdgpu_buffer_rsrc_t rs, int idx)
 // [Plo, x+1-K].  Only the first candidate can start left of Dlo.  Dhc is the bin's right
 // end clipped to the line's L*.  Advances x and Plo.
 template <int ARITH>
-__device__ __forceinline__ float cand_step(bool first, bool in_range, float& x, float& Plo,
-                                           float K, float Dlo, float Dhc, float val, float acc)
+__device__ __forceinline__ float cand_step(bool first, float& x, float& Plo, float K,
+                                           float Dlo, float Dhc, float val, float acc)
 {
     x = __fadd_rn(x, 1.0f);
     const float Phi = __fsub_rn(x, K);
-    const float R = (Phi < Dhc) ? Phi : Dhc;
-    const float L = first ? ((Plo > Dlo) ? Plo : Dlo) : Plo;
-    const bool ok = in_range && R > L;
+    const float R = fminf(Phi, Dhc);
+    const float L = first ? fmaxf(Plo, Dlo) : Plo;
     const float nacc = accum<ARITH>(__fsub_rn(R, L), val, acc);
     Plo = Phi;
-    return ok ? nacc : acc;
+    return R > L ? nacc : acc;
+}
+
+// (index - nh) of the pixel whose span [P_q, P_q+1) holds Dlo on a line with offset K,
+// clamped into [0, npix): a floor estimate, then one exact correction against the keys.
+__device__ __forceinline__ float first_pixel(float Dlo, float K, int nh, int npix)
+{
+    // the estimate needs no clamp of its own: far outside the line the correction may
+    // be inexact, but the final clamp then lands on the end pixel either way
+    float fe = floorf(__fadd_rn(Dlo, K));
+    const float Pe = __fsub_rn(fe, K);
+    const float Pe1 = __fsub_rn(__fadd_rn(fe, 1.0f), K);
+    // selects, not branches: at most one of the two corrections applies
+    const float up = (Dlo >= Pe1) ? 1.0f : 0.0f;
+    const float dn = (Dlo < Pe) ? 1.0f : 0.0f;
+    fe = __fsub_rn(__fadd_rn(fe, up), dn);
+    return fminf(fmaxf(fe, (float)(-nh)), (float)(npix - 1 - nh));
 }
 
 // Forward projection: one thread per (bin, view blockIdx.y, slice blockIdx.z).
 // Lines are image rows for row sweeps and image columns for column sweeps; column
 // sweeps (and row sums) read the transposed image so that the lanes of a wave, which
 // hold consecutive bins, read consecutive addresses.
-// Bins [bx*kDdFwdBlock, +kDdFwdBlock) of view `view`, slice z; blk indexes the partials.
-template <int ARITH, int EPI, bool ONES>
+// Bins [bx*B, +B) of view `view`, slice z, one per thread; blk indexes the partials.
+template <int ARITH, int EPI, bool ONES, int B>
 __device__ __forceinline__ void dd_fwd_block(const DdView* __restrict__ geom,
                                              const float* __restrict__ lstar, int nang,
                                              int npix, const float* __restrict__ image,
@@ -333,7 +371,7 @@ __device__ __forceinline__ void dd_fwd_block(const DdView* __restrict__ geom,
                                              int64_t sino_stride, int accumulate,
                                              const EpiArgs& e, int bx, int view, int z, int blk)
 {
-    const int d = bx * kDdFwdBlock + threadIdx.x;
+    const int d = bx * B + threadIdx.x;
     const bool active = d < npix;
     const DdView g = geom[view];
     image += (int64_t)z * img_stride;
@@ -362,8 +400,8 @@ __device__ __forceinline__ void dd_fwd_block(const DdView* __restrict__ geom,
             const float* ls = lstar + (int64_t)view * npix;
             const float x0 = (float)(-nh), xN = (float)(npix - nh);
             // lines whose pixel span [x0 - K, xN - K] can meet [Dlo, Dhi]:
-            // x0 - Dhi < K_i < xN - Dlo with K_i ~ (i - nh + 0.5) * t; two lines of margin,
-            // the exact test below decides.
+            // x0 - Dhi < K_i < xN - Dlo with K_i ~ (i - nh + 0.5) * t, two lines of margin
+            // (margin lines that miss the bin add nothing).
             double ua = ((double)x0 - (double)Dhi) / (double)g.t;
             double ub = ((double)xN - (double)Dlo) / (double)g.t;
             if (ua > ub) { const double tmp = ua; ua = ub; ub = tmp; }
@@ -371,36 +409,40 @@ __device__ __forceinline__ void dd_fwd_block(const DdView* __restrict__ geom,
             ub = fmin(fmax(ub + nh - 0.5, -4.0), (double)npix + 4.0);
             const int i_lo = max(0, (int)floor(ua) - 2);
             const int i_hi = min(npix - 1, (int)ceil(ub) + 2);
+            // Lines outside the bin, and candidates past the bin or past the line's end,
+            // need no test: the first pixel is clamped into [0, npix), and every weight's
+            // right end is clipped to L* <= P[N], so such candidates have R <= L.
+            // L* >= min(P[N-1], D[N-1]) on every line, so the clip (and the table read)
+            // is needed only where Dhi exceeds that bound: the line's last pixel or the
+            // last bin.
             const int nc = g.ncand;
-            for (int i = i_lo; i <= i_hi; ++i) {
-                const float K = key_K<ARITH>(g, line_coord(i, nh));
-                const float P0 = __fsub_rn(x0, K), PN = __fsub_rn(xN, K);
-                if (!(PN > Dlo && P0 < Dhi)) continue;
-                const float Dhc = fminf(Dhi, ls[i]);
-                // the pixel whose span holds Dlo: estimate, then one exact correction
-                float fe = floorf(__fadd_rn(Dlo, K));
-                fe = fminf(fmaxf(fe, (float)(-nh - 1)), (float)(npix - nh));
-                int q = (int)fe + nh;  // in [-1, npix]
-                {
-                    const float Pe = __fsub_rn((float)(q - nh), K);
-                    const float Pe1 = __fsub_rn((float)(q + 1 - nh), K);
-                    q += (Dlo >= Pe1) ? 1 : ((Dlo < Pe) ? -1 : 0);
-                    // a bin starting before pixel 0 first meets pixel 0
-                    q = min(max(q, 0), npix - 1);
-                }
-                // candidates q .. q+nc-1; loads go through a buffer descriptor whose
-                // range check returns 0 outside the image, so no index clamping
-                const int lbase = i * npix;
-                float x = (float)(q - nh);
-                float Plo = __fsub_rn(x, K);
-                if (nc <= 3) {
+            const bool last_bin = d == npix - 1;
+            const float xNm1 = (float)(npix - 1 - nh);
+            float u = line_coord(i_lo, nh);
+            int lbase = i_lo * npix;
+            if (nc <= 3) {
+                for (int i = i_lo; i <= i_hi; ++i, u = __fadd_rn(u, 1.0f), lbase += npix) {
+                    const float K = key_K<ARITH>(g, u);
+                    const float x = first_pixel(Dlo, K, nh, npix);
+                    const int q = (int)x + nh;
+                    float Dhc = Dhi;
+                    if (last_bin || Dhi > __fsub_rn(xNm1, K)) Dhc = fminf(Dhi, ls[i]);
                     float v[3];
 #pragma unroll
                     for (int k = 0; k < 3; ++k) v[k] = ONES ? 1.0f : load_px(rs, lbase + q + k);
+                    float xk = x, Plo = __fsub_rn(x, K);
 #pragma unroll
                     for (int k = 0; k < 3; ++k)
-                        acc = cand_step<ARITH>(k == 0, q + k < npix, x, Plo, K, Dlo, Dhc, v[k], acc);
-                } else {
+                        acc = cand_step<ARITH>(k == 0, xk, Plo, K, Dlo, Dhc, v[k], acc);
+                }
+            } else {
+                for (int i = i_lo; i <= i_hi; ++i, u = __fadd_rn(u, 1.0f), lbase += npix) {
+                    const float K = key_K<ARITH>(g, u);
+                    const float x = first_pixel(Dlo, K, nh, npix);
+                    const int q = (int)x + nh;
+                    float Dhc = Dhi;
+                    if (last_bin || Dhi > __fsub_rn(xNm1, K)) Dhc = fminf(Dhi, ls[i]);
+                    float xk = x, Plo = __fsub_rn(x, K);
                     for (int k0 = 0; k0 < nc; k0 += 4) {
                         float v[4];
 #pragma unroll
@@ -408,8 +450,7 @@ __device__ __forceinline__ void dd_fwd_block(const DdView* __restrict__ geom,
                             v[k] = ONES ? 1.0f : load_px(rs, lbase + q + k0 + k);
 #pragma unroll
                         for (int k = 0; k < 4; ++k)
-                            acc = cand_step<ARITH>(k0 + k == 0, (k0 + k < nc) && (q + k0 + k < npix),
-                                                   x, Plo, K, Dlo, Dhc, v[k], acc);
+                            acc = cand_step<ARITH>(k0 + k == 0, xk, Plo, K, Dlo, Dhc, v[k], acc);
                     }
                 }
             }

@@ -58,6 +58,8 @@ struct Plan {
     void* d_geom;    // device array of DdView / RdView
     void* h_geom;    // host copy
     float* d_lstar;  // distance-driven: [view][line] start of the dropped segment
+    int fwd_var;     // distance-driven launch shapes (kFwdBins / kBackTiles index)
+    int back_var;
     // Device scratch owned by the plan (FP64 partial sums, ray-driven back-projection
     // image).  Grown on first use; a plan must not be used from two streams at once.
     mutable void* scratch;
@@ -68,25 +70,41 @@ struct Plan {
 int plan_scratch(const Plan& p, size_t bytes, void** ptr, hipStream_t st);
 
 // Launch geometry shared by the launchers and the scratch layout.
-constexpr int kDdFwdBlock = 256;            // bins per forward block
-constexpr int kDdBackTile = 64;             // back block = 64x64 pixel tile
+// Distance-driven launch shapes (every shape gives bit-identical projections):
+//   forward: kFwdBins[v] bins of one view per block, one bin per thread;
+//   back:    kBackTiles[v] = {T, P}: T x T pixel tile per block, P x P per thread.
+constexpr int kNumFwdVariants = 3;
+constexpr int kFwdBins[kNumFwdVariants] = {256, 128, 64};
+constexpr int kNumBackVariants = 3;
+constexpr int kBackTiles[kNumBackVariants][2] = {{64, 4}, {32, 2}, {32, 4}};
+constexpr int kDdFwdBlock = 256;            // default forward shape (also the batch's)
+constexpr int kDdBackTile = 64;             // default back tile (also the batch's)
 constexpr int kRdBlock = 64;
 constexpr int kEpiChunk = 256 * 8;          // pixels per elementwise-epilogue block
+
+inline size_t dd_fwd_blocks(int npix, int nang, int bins)
+{
+    return (size_t)((npix + bins - 1) / bins) * nang;
+}
+
+inline size_t dd_back_blocks(int npix, int tile)
+{
+    const size_t t = (size_t)((npix + tile - 1) / tile);
+    return t * t;
+}
 
 inline size_t fwd_blocks_per_slice(const Plan& p)
 {
     return p.projector == ITOMO_PROJ_DD
-               ? (size_t)((p.npix + kDdFwdBlock - 1) / kDdFwdBlock) * p.nang
+               ? dd_fwd_blocks(p.npix, p.nang, kFwdBins[p.fwd_var])
                : (size_t)((p.npix + kRdBlock - 1) / kRdBlock) * p.nang;
 }
 
 inline size_t back_blocks_per_slice(const Plan& p)
 {
     const size_t n2 = (size_t)p.npix * p.npix;
-    return p.projector == ITOMO_PROJ_DD
-               ? (size_t)((p.npix + kDdBackTile - 1) / kDdBackTile) *
-                     (size_t)((p.npix + kDdBackTile - 1) / kDdBackTile)
-               : (n2 + kEpiChunk - 1) / kEpiChunk;
+    return p.projector == ITOMO_PROJ_DD ? dd_back_blocks(p.npix, kBackTiles[p.back_var][0])
+                                        : (n2 + kEpiChunk - 1) / kEpiChunk;
 }
 
 // Per-call device scratch carved from the plan's buffer.  The layout depends only on

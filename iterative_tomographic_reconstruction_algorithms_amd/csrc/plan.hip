@@ -258,6 +258,28 @@ int itomo_plan_trig(const itomo_plan* p, float* s_host, float* c_host)
     return ITOMO_OK;
 }
 
+int itomo_plan_set_shapes(itomo_plan* p, int fwd_variant, int back_variant)
+{
+    if (!p) return set_error(ITOMO_EINVAL, "plan_set_shapes: null plan");
+    const int nf = p->projector == ITOMO_PROJ_DD ? kNumFwdVariants : 1;
+    const int nb = p->projector == ITOMO_PROJ_DD ? kNumBackVariants : 1;
+    if (fwd_variant < 0 || fwd_variant >= nf || back_variant < 0 || back_variant >= nb)
+        return set_error(ITOMO_EINVAL, "plan_set_shapes: variants (%d, %d) outside [0, %d) x "
+                                       "[0, %d)", fwd_variant, back_variant, nf, nb);
+    p->fwd_var = fwd_variant;
+    p->back_var = back_variant;
+    return ITOMO_OK;
+}
+
+int itomo_plan_get_shapes(const itomo_plan* p, int* fwd_variant, int* back_variant)
+{
+    if (!p || !fwd_variant || !back_variant)
+        return set_error(ITOMO_EINVAL, "plan_get_shapes: null pointer");
+    *fwd_variant = p->fwd_var;
+    *back_variant = p->back_var;
+    return ITOMO_OK;
+}
+
 static int check_io(const itomo_plan* p, const void* a, const void* b, int nslices,
                     const char* who)
 {
