@@ -74,6 +74,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--residual-target", type=float, default=1e-3)
+    ap.add_argument("--pmc-calibration", action="store_true",
+                    help="after the timed steps, copy a 256 MiB float32 array 3 times "
+                         "(a kernel of known traffic for scaling FETCH_SIZE / WRITE_SIZE)")
     ap.add_argument("--max-residual-iters", type=int, default=0,
                     help="also time plain SIRT from zero to the residual target (0: skip)")
     args = ap.parse_args()
@@ -157,6 +160,15 @@ def main():
         "kernels_ms": {"sirt_forward": round(fwd_ms, 4), "sirt_back": round(back_ms, 4)},
         "residual_after_timed_steps": resid_rel,
     }
+
+    if args.pmc_calibration:
+        src = torch.rand(64 << 20, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        line["pmc_calibration"] = {"copy_bytes_read": src.numel() * 4,
+                                   "copy_bytes_written": dst.numel() * 4}
 
     if args.max_residual_iters > 0:
         x.zero_()

@@ -244,6 +244,58 @@ int itomo_filter_rows(const float *in, float *out, int nrows, int npix, const do
 int itomo_dot_f64(const float *a, const float *b, int64_t n, double *out, double *work,
                   void *stream);
 
+/* ---------------------------------------------------------------------------------
+ * Learned warm start: kernels of a small U-Net (no counterpart in the reference, which
+ * has no learned component).  Tensors are DEVICE arrays in NHWC order; x, y, wt, mask,
+ * and every other argument declared void* (except stream) hold BF16 (16-bit, round to nearest
+ * even); float/double pointers are FP32/FP64.  Every product accumulates in FP32 on the
+ * matrix cores (v_mfma_f32_16x16x32_bf16).
+ * itomo_unet_conv3x3: y[p][co] = act(bias[co] + sum_{tap,ci} x[p+off(tap)][ci] *
+ *   wt[co][tap][ci]) with zero padding, act = ReLU when relu != 0, then multiplied by
+ *   (mask[p][co] > 0) when mask is non-null.  bias may be null.
+ * itomo_unet_flip: wf[ci][tap][co] = wt[co][8-tap][ci], the weights that turn conv3x3
+ *   into the gradient with respect to its input.
+ * itomo_unet_wgrad: dw[co][tap][ci] += sum_p dy[p][co] x[p+off(tap)][ci] and, when db is
+ *   non-null, db[co] += sum_p dy[p][co] (FP32 atomics: summation order not fixed).
+ * itomo_unet_pool_fwd: 2x2 max pool (h, w even), idx (uint8) = argmax (first on ties).
+ * itomo_unet_pool_bwd: dx = (skip + dy scattered to the argmax) * (xm > 0); skip may
+ *   be null.
+ * itomo_unet_upcat_fwd: y[n][y][x] = concat(a[n][y/2][x/2][0:ca], b[n][y][x][0:cb]).
+ * itomo_unet_upcat_bwd: da = (2x2 sum of dy[.., 0:ca]) * (a > 0), db = dy[.., ca:].
+ * itomo_unet_head_fwd: out[p] = img[p] + hb[0] + sum_c hw[c] f[p][c] (c <= 32).
+ * itomo_unet_head_bwd: df[p][c] = dout[p] hw[c] (f > 0); dhw += sum_p dout f;
+ *   dhb += sum_p dout.
+ * itomo_unet_mse: grad = 2 (out - target) / n; loss[0] = mean (out - target)^2 (FP64);
+ *   work: ITOMO_UNET_MSE_WORK doubles.
+ * itomo_unet_adam: Adam with bias correction (step >= 1) on FP32 w; refreshes the BF16
+ *   copy wbf when non-null.
+ * itomo_unet_to_bf16: y = bf16(x).
+ * ------------------------------------------------------------------------------- */
+#define ITOMO_UNET_MSE_WORK 1024
+int itomo_unet_conv3x3(const void *x, int n, int h, int w, int cin, const void *wt,
+                       const float *bias, int cout, const void *mask, int relu, void *y,
+                       void *stream);
+int itomo_unet_flip(const void *wt, int cin, int cout, void *wf, void *stream);
+int itomo_unet_wgrad(const void *x, const void *dy, int n, int h, int w, int cin, int cout,
+                     float *dw, float *db, void *stream);
+int itomo_unet_pool_fwd(const void *x, int n, int h, int w, int c, void *y, void *idx,
+                        void *stream);
+int itomo_unet_pool_bwd(const void *dy, const void *idx, const void *skip, const void *xm,
+                        int n, int h, int w, int c, void *dx, void *stream);
+int itomo_unet_upcat_fwd(const void *a, const void *b, int n, int h, int w, int ca, int cb,
+                         void *y, void *stream);
+int itomo_unet_upcat_bwd(const void *dy, const void *a, int n, int h, int w, int ca, int cb,
+                         void *da, void *db, void *stream);
+int itomo_unet_head_fwd(const void *f, const float *hw, const float *hb, const float *img,
+                        int64_t npixels, int c, float *out, void *stream);
+int itomo_unet_head_bwd(const void *f, const float *hw, const float *dout, int64_t npixels,
+                        int c, void *df, float *dhw, float *dhb, void *stream);
+int itomo_unet_mse(const float *out, const float *target, int64_t npixels, float *grad,
+                   double *loss, double *work, void *stream);
+int itomo_unet_adam(float *w, const float *g, float *m, float *v, void *wbf, int64_t n,
+                    float lr, float beta1, float beta2, float eps, int step, void *stream);
+int itomo_unet_to_bf16(const float *x, int64_t n, void *y, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

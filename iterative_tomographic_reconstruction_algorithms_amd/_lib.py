@@ -47,6 +47,18 @@ _SIGS = {
     "itomo_mlem_back": ([_vp, _vp, _vp, _vp, _i, _vp, _vp], _i),
     "itomo_mlem_iterate": ([_vp, _vp, _vp, _vp, _vp, _i, _vp, _vp], _i),
     "itomo_dot_f64": ([_vp, _vp, _i64, _vp, _vp, _vp], _i),
+    "itomo_unet_conv3x3": ([_vp, _i, _i, _i, _i, _vp, _vp, _i, _vp, _i, _vp, _vp], _i),
+    "itomo_unet_flip": ([_vp, _i, _i, _vp, _vp], _i),
+    "itomo_unet_wgrad": ([_vp, _vp, _i, _i, _i, _i, _i, _vp, _vp, _vp], _i),
+    "itomo_unet_pool_fwd": ([_vp, _i, _i, _i, _i, _vp, _vp, _vp], _i),
+    "itomo_unet_pool_bwd": ([_vp, _vp, _vp, _vp, _i, _i, _i, _i, _vp, _vp], _i),
+    "itomo_unet_upcat_fwd": ([_vp, _vp, _i, _i, _i, _i, _i, _vp, _vp], _i),
+    "itomo_unet_upcat_bwd": ([_vp, _vp, _i, _i, _i, _i, _i, _vp, _vp, _vp], _i),
+    "itomo_unet_head_fwd": ([_vp, _vp, _vp, _vp, _i64, _i, _vp, _vp], _i),
+    "itomo_unet_head_bwd": ([_vp, _vp, _vp, _i64, _i, _vp, _vp, _vp, _vp], _i),
+    "itomo_unet_mse": ([_vp, _vp, _i64, _vp, _vp, _vp, _vp], _i),
+    "itomo_unet_adam": ([_vp, _vp, _vp, _vp, _vp, _i64, _f, _f, _f, _f, _i, _vp], _i),
+    "itomo_unet_to_bf16": ([_vp, _i64, _vp, _vp], _i),
     "itomo_sps_weights": ([_vp, _vp, _i, _vp, _vp, _vp, _vp, _vp], _i),
     "itomo_sps_iterate": ([_vp, _vp, _vp, _vp, _vp, _vp, _i, _vp, _vp], _i),
     "itomo_grad": ([_vp, _i, _i, _vp, _vp], _i),
@@ -69,6 +81,7 @@ _SIGS = {
     "itomo_filter_rows": ([_vp, _vp, _i, _i, _vp, _vp], _i),
 }
 
+UNET_MSE_WORK = 1024  # ITOMO_UNET_MSE_WORK
 FWD_VARIANTS = 3      # ITOMO_FWD_VARIANTS
 BACK_VARIANTS = 3     # ITOMO_BACK_VARIANTS
 FILTERS = {"ramp": 0, "shepp-logan": 1, "cosine": 2, "hamming": 3, "hanning": 4, "none": 5}

@@ -357,6 +357,8 @@ __device__ __forceinline__ float first_pixel(float Dlo, float K, int nh, int npi
     return fminf(fmaxf(fe, (float)(-nh)), (float)(npix - 1 - nh));
 }
 
+constexpr int kFwdLines = 4;   // lines whose gathers are in flight together
+
 // Forward projection: one thread per (bin, view blockIdx.y, slice blockIdx.z).
 // Lines are image rows for row sweeps and image columns for column sweeps; column
 // sweeps (and row sums) read the transposed image so that the lanes of a wave, which
@@ -421,7 +423,34 @@ __device__ __forceinline__ void dd_fwd_block(const DdView* __restrict__ geom,
             float u = line_coord(i_lo, nh);
             int lbase = i_lo * npix;
             if (nc <= 3) {
-                for (int i = i_lo; i <= i_hi; ++i, u = __fadd_rn(u, 1.0f), lbase += npix) {
+                // Lines in groups of kFwdLines: all their loads are issued before the first
+                // is consumed, so a wave keeps several gathers in flight; the lines are
+                // then accumulated in increasing order as before.
+                int i = i_lo;
+                for (; i + kFwdLines - 1 <= i_hi; i += kFwdLines) {
+                    float K[kFwdLines], xq[kFwdLines], v[kFwdLines][3];
+#pragma unroll
+                    for (int l = 0; l < kFwdLines; ++l) {
+                        K[l] = key_K<ARITH>(g, u);
+                        u = __fadd_rn(u, 1.0f);
+                        xq[l] = first_pixel(Dlo, K[l], nh, npix);
+                        const int q = (int)xq[l] + nh;
+#pragma unroll
+                        for (int k = 0; k < 3; ++k)
+                            v[l][k] = ONES ? 1.0f : load_px(rs, lbase + q + k);
+                        lbase += npix;
+                    }
+#pragma unroll
+                    for (int l = 0; l < kFwdLines; ++l) {
+                        float Dhc = Dhi;
+                        if (last_bin || Dhi > __fsub_rn(xNm1, K[l])) Dhc = fminf(Dhi, ls[i + l]);
+                        float xk = xq[l], Plo = __fsub_rn(xq[l], K[l]);
+#pragma unroll
+                        for (int k = 0; k < 3; ++k)
+                            acc = cand_step<ARITH>(k == 0, xk, Plo, K[l], Dlo, Dhc, v[l][k], acc);
+                    }
+                }
+                for (; i <= i_hi; ++i, u = __fadd_rn(u, 1.0f), lbase += npix) {
                     const float K = key_K<ARITH>(g, u);
                     const float x = first_pixel(Dlo, K, nh, npix);
                     const int q = (int)x + nh;

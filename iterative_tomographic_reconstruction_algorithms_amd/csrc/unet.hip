@@ -1,0 +1,677 @@
+// Learned warm start: the kernels of a small U-Net (3x3 convolutions, ReLU, 2x2 max
+// pooling, nearest-neighbour upsampling with skip concatenation, a 1x1 head added to
+// the input image), forward and backward, plus the MSE loss and Adam.
+//
+// Activations and weights are BF16 (raw 16-bit storage, round to nearest even), every
+// product is accumulated in FP32 on the matrix cores: a 3x3 convolution is an implicit
+// GEMM, M = pixels, N = output channels, K = 9 * input channels, evaluated with
+// v_mfma_f32_16x16x32_bf16.  Layout is NHWC (channels fastest) so that the 8 K-values a
+// lane feeds to one MFMA are 8 consecutive channels of one pixel and one tap: a single
+// 16-byte load.  Master weights, gradients and Adam moments are FP32.
+//
+// Operand layout of v_mfma_f32_16x16x32_bf16 (64 lanes, l = lane):
+//   A (16 x 32, rows i, K k):  lane l holds A[l % 16][8 * (l / 16) + e], e = 0..7
+//   B (32 x 16, K k, cols j):  lane l holds B[8 * (l / 16) + e][l % 16]
+//   D (16 x 16):               lane l holds D[4 * (l / 16) + r][l % 16], r = 0..3
+#include "itomo_internal.h"
+
+namespace itomo {
+namespace {
+
+typedef unsigned short bf16_t;
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+union Frag {
+    uint4 u;
+    bf16x8 v;
+};
+
+__device__ __forceinline__ float bf2f(bf16_t h) { return __uint_as_float((unsigned)h << 16); }
+
+__device__ __forceinline__ bf16_t f2bf(float f)
+{
+    unsigned u = __float_as_uint(f);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return (bf16_t)0x7fc0;  // NaN
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (bf16_t)(u >> 16);
+}
+
+__device__ __forceinline__ bf16x8 zero8()
+{
+    Frag f;
+    f.u = make_uint4(0, 0, 0, 0);
+    return f.v;
+}
+
+__device__ __forceinline__ bf16x8 load8(const bf16_t* p)
+{
+    Frag f;
+    f.u = *(const uint4*)p;
+    return f.v;
+}
+
+__device__ __forceinline__ f32x4 mfma(bf16x8 a, bf16x8 b, f32x4 c)
+{
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+// ---- 3x3 convolution, stride 1, zero padding 1 -------------------------------------
+//
+// Y[p][co] = act(bias[co] + sum_{tap, ci} X[p + off(tap)][ci] * Wt[co][tap][ci]),
+// optionally multiplied by (mask[p][co] > 0).  Block = 4 waves = 128 pixels x 32 output
+// channels; each wave a 32 x 32 tile (2 x 2 MFMA tiles).  VEC: Cin % 8 == 0, so a lane's
+// 8 K-values share one tap and are one 16-byte load; otherwise element loads.
+template <bool VEC>
+__global__ void __launch_bounds__(256)
+conv3x3_kernel(const bf16_t* __restrict__ X, const bf16_t* __restrict__ Wt,
+               const float* __restrict__ bias, const bf16_t* __restrict__ mask,
+               bf16_t* __restrict__ Y, int N, int H, int W, int Cin, int Cout, int relu)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int li = lane & 15, lg = lane >> 4;
+    const int64_t P = (int64_t)N * H * W;
+    const int64_t p0 = (int64_t)blockIdx.x * 128 + wave * 32;
+    const int co0 = blockIdx.y * 32;
+
+    int ya[2], xa[2];
+    int64_t rowa[2];  // (n*H) * W base of the pixel's image
+    bool va[2];
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+        const int64_t p = p0 + t * 16 + li;
+        va[t] = p < P;
+        const int64_t pp = va[t] ? p : 0;
+        const int64_t hw = (int64_t)H * W;
+        const int64_t n = pp / hw;
+        const int64_t r = pp - n * hw;
+        ya[t] = (int)(r / W);
+        xa[t] = (int)(r - (int64_t)ya[t] * W);
+        rowa[t] = n * hw;
+    }
+    f32x4 acc[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+    const int K = 9 * Cin;
+    for (int k0 = 0; k0 < K; k0 += 32) {
+        const int kk = k0 + 8 * lg;
+        bf16x8 a[2], b[2];
+        if (VEC) {
+            const bool kin = kk < K;
+            const int tap = kin ? kk / Cin : 0, ci = kk - tap * Cin;
+            const int dy = tap / 3 - 1, dx = tap % 3 - 1;
+#pragma unroll
+            for (int t = 0; t < 2; ++t) {
+                const int y = ya[t] + dy, x = xa[t] + dx;
+                const bool ok = kin && va[t] && y >= 0 && y < H && x >= 0 && x < W;
+                a[t] = ok ? load8(X + (rowa[t] + (int64_t)y * W + x) * Cin + ci) : zero8();
+            }
+#pragma unroll
+            for (int t = 0; t < 2; ++t) {
+                const int co = co0 + t * 16 + li;
+                b[t] = (kin && co < Cout) ? load8(Wt + ((int64_t)co * 9 + tap) * Cin + ci)
+                                          : zero8();
+            }
+        } else {
+#pragma unroll
+            for (int t = 0; t < 2; ++t) {
+                bf16_t ea[8], eb[8];
+                const int co = co0 + t * 16 + li;
+#pragma unroll
+                for (int e = 0; e < 8; ++e) {
+                    const int k = kk + e;
+                    ea[e] = 0;
+                    eb[e] = 0;
+                    if (k < K) {
+                        const int tap = k / Cin, ci = k - tap * Cin;
+                        const int y = ya[t] + tap / 3 - 1, x = xa[t] + tap % 3 - 1;
+                        if (va[t] && y >= 0 && y < H && x >= 0 && x < W)
+                            ea[e] = X[(rowa[t] + (int64_t)y * W + x) * Cin + ci];
+                        if (co < Cout) eb[e] = Wt[(int64_t)co * K + k];
+                    }
+                }
+                Frag fa, fb;
+                fa.u = make_uint4(ea[0] | ((unsigned)ea[1] << 16), ea[2] | ((unsigned)ea[3] << 16),
+                                  ea[4] | ((unsigned)ea[5] << 16), ea[6] | ((unsigned)ea[7] << 16));
+                fb.u = make_uint4(eb[0] | ((unsigned)eb[1] << 16), eb[2] | ((unsigned)eb[3] << 16),
+                                  eb[4] | ((unsigned)eb[5] << 16), eb[6] | ((unsigned)eb[7] << 16));
+                a[t] = fa.v;
+                b[t] = fb.v;
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j) acc[i][j] = mfma(a[i], b[j], acc[i][j]);
+    }
+
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            const int co = co0 + j * 16 + li;
+            if (co >= Cout) continue;
+            const float bv = bias ? bias[co] : 0.0f;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int64_t p = p0 + i * 16 + 4 * lg + r;
+                if (p >= P) continue;
+                float v = acc[i][j][r] + bv;
+                if (relu) v = fmaxf(v, 0.0f);
+                const int64_t o = p * Cout + co;
+                if (mask && !(bf2f(mask[o]) > 0.0f)) v = 0.0f;
+                Y[o] = f2bf(v);
+            }
+        }
+}
+
+// Wf[ci][tap][co] = Wt[co][8 - tap][ci]: the weights of the transposed convolution
+// (gradient with respect to the input of a 3x3, pad-1 convolution).
+__global__ void flip_kernel(const bf16_t* __restrict__ Wt, bf16_t* __restrict__ Wf, int Cin,
+                            int Cout)
+{
+    const int64_t n = (int64_t)Cout * 9 * Cin;
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n;
+         k += (int64_t)gridDim.x * blockDim.x) {
+        const int co = (int)(k % Cout);
+        const int64_t r = k / Cout;
+        const int tap = (int)(r % 9);
+        const int ci = (int)(r / 9);
+        Wf[k] = Wt[((int64_t)co * 9 + (8 - tap)) * Cin + ci];
+    }
+}
+
+// dW[co][tap][ci] += sum_p dY[p][co] * X[p + off(tap)][ci] over the pixels
+// [blockIdx.z * chunk, +chunk).  GEMM M = Cout, N = 9*Cin, K = pixels; block = 64 x 64
+// outputs (4 waves, 2 x 2 of 32 x 32), 32 pixels per step staged transposed in LDS so
+// that each lane's 8 K-values (8 consecutive pixels) are one 16-byte LDS read.
+__global__ void __launch_bounds__(256)
+wgrad_kernel(const bf16_t* __restrict__ X, const bf16_t* __restrict__ dY,
+             float* __restrict__ dW, int N, int H, int W, int Cin, int Cout, int64_t chunk)
+{
+    __shared__ __attribute__((aligned(16))) bf16_t sA[64][40];  // [co][pixel]
+    __shared__ __attribute__((aligned(16))) bf16_t sB[64][40];  // [col][pixel]
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int li = lane & 15, lg = lane >> 4;
+    const int wr = wave >> 1, wc = wave & 1;
+    const int ncol = 9 * Cin;
+    const int col0 = blockIdx.x * 64, co0 = blockIdx.y * 64;
+    const int64_t P = (int64_t)N * H * W, hw = (int64_t)H * W;
+    const int64_t pb = (int64_t)blockIdx.z * chunk;
+    const int64_t pe = min(P, pb + chunk);
+
+    f32x4 acc[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+    // staging role: pixel sp of the 32, channel/column group sg*8 .. +7
+    const int sp = tid >> 3, sg = (tid & 7) * 8;
+    for (int64_t ps = pb; ps < pe; ps += 32) {
+        const int64_t p = ps + sp;
+        const bool pin = p < pe;
+        // dY[p][co0 + sg .. +7] -> sA[sg + e][sp]
+        {
+            bf16_t v[8];
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                const int co = co0 + sg + e;
+                v[e] = (pin && co < Cout) ? dY[p * Cout + co] : (bf16_t)0;
+            }
+#pragma unroll
+            for (int e = 0; e < 8; ++e) sA[sg + e][sp] = v[e];
+        }
+        // X[p + off(tap)][ci] for columns col0 + sg .. +7 -> sB[sg + e][sp]
+        {
+            int64_t n = 0;
+            int y = 0, x = 0;
+            if (pin) {
+                n = p / hw;
+                const int64_t r = p - n * hw;
+                y = (int)(r / W);
+                x = (int)(r - (int64_t)y * W);
+            }
+            bf16_t v[8];
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                const int col = col0 + sg + e;
+                v[e] = 0;
+                if (pin && col < ncol) {
+                    const int tap = col / Cin, ci = col - tap * Cin;
+                    const int yy = y + tap / 3 - 1, xx = x + tap % 3 - 1;
+                    if (yy >= 0 && yy < H && xx >= 0 && xx < W)
+                        v[e] = X[((n * H + yy) * W + xx) * Cin + ci];
+                }
+            }
+#pragma unroll
+            for (int e = 0; e < 8; ++e) sB[sg + e][sp] = v[e];
+        }
+        __syncthreads();
+        bf16x8 a[2], b[2];
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+            a[t] = load8(&sA[wr * 32 + t * 16 + li][8 * lg]);
+            b[t] = load8(&sB[wc * 32 + t * 16 + li][8 * lg]);
+        }
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j) acc[i][j] = mfma(a[i], b[j], acc[i][j]);
+        __syncthreads();
+    }
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            const int col = col0 + wc * 32 + j * 16 + li;
+            if (col >= ncol) continue;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int co = co0 + wr * 32 + i * 16 + 4 * lg + r;
+                if (co < Cout) atomicAdd(&dW[(int64_t)co * ncol + col], acc[i][j][r]);
+            }
+        }
+}
+
+// db[c] += sum over the block's pixels of dY[p][c]; thread c of a block of C threads.
+__global__ void bias_grad_kernel(const bf16_t* __restrict__ dY, float* __restrict__ db,
+                                 int64_t P, int C, int64_t chunk)
+{
+    const int c = threadIdx.x;
+    if (c >= C) return;
+    const int64_t pb = (int64_t)blockIdx.x * chunk, pe = min(P, pb + chunk);
+    float s = 0.0f;
+    for (int64_t p = pb; p < pe; ++p) s += bf2f(dY[p * C + c]);
+    atomicAdd(&db[c], s);
+}
+
+// ---- pooling, upsampling, concatenation ---------------------------------------------
+
+// Y = 2x2 max pool of X (H, W even); idx = position of the maximum (first on ties).
+__global__ void pool_fwd_kernel(const bf16_t* __restrict__ X, bf16_t* __restrict__ Y,
+                                unsigned char* __restrict__ idx, int N, int H, int W, int C)
+{
+    const int Ho = H / 2, Wo = W / 2;
+    const int64_t n_out = (int64_t)N * Ho * Wo * C;
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n_out;
+         k += (int64_t)gridDim.x * blockDim.x) {
+        const int c = (int)(k % C);
+        int64_t r = k / C;
+        const int x = (int)(r % Wo);
+        r /= Wo;
+        const int y = (int)(r % Ho);
+        const int64_t n = r / Ho;
+        float best = 0.0f;
+        int bi = 0;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int yy = 2 * y + (q >> 1), xx = 2 * x + (q & 1);
+            const float v = bf2f(X[((n * H + yy) * W + xx) * C + c]);
+            if (q == 0 || v > best) {
+                best = v;
+                bi = q;
+            }
+        }
+        Y[k] = f2bf(best);
+        idx[k] = (unsigned char)bi;
+    }
+}
+
+// dX = (skip + scatter of dY to the argmax) * (Xm > 0); skip may be null.
+__global__ void pool_bwd_kernel(const bf16_t* __restrict__ dY, const unsigned char* __restrict__ idx,
+                                const bf16_t* __restrict__ skip, const bf16_t* __restrict__ Xm,
+                                bf16_t* __restrict__ dX, int N, int H, int W, int C)
+{
+    const int Ho = H / 2, Wo = W / 2;
+    const int64_t n_in = (int64_t)N * H * W * C;
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n_in;
+         k += (int64_t)gridDim.x * blockDim.x) {
+        const int c = (int)(k % C);
+        int64_t r = k / C;
+        const int x = (int)(r % W);
+        r /= W;
+        const int y = (int)(r % H);
+        const int64_t n = r / H;
+        const int64_t o = ((n * Ho + y / 2) * Wo + x / 2) * C + c;
+        float g = skip ? bf2f(skip[k]) : 0.0f;
+        if (idx[o] == (unsigned char)((y & 1) * 2 + (x & 1))) g += bf2f(dY[o]);
+        if (!(bf2f(Xm[k]) > 0.0f)) g = 0.0f;
+        dX[k] = f2bf(g);
+    }
+}
+
+// Y[n][y][x] = concat(A[n][y/2][x/2][0:Ca], B[n][y][x][0:Cb]); Y has H x W, A H/2 x W/2.
+__global__ void upcat_fwd_kernel(const bf16_t* __restrict__ A, const bf16_t* __restrict__ B,
+                                 bf16_t* __restrict__ Y, int N, int H, int W, int Ca, int Cb)
+{
+    const int C = Ca + Cb;
+    const int64_t n_out = (int64_t)N * H * W * C;
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n_out;
+         k += (int64_t)gridDim.x * blockDim.x) {
+        const int c = (int)(k % C);
+        const int64_t pix = k / C;
+        const int x = (int)(pix % W);
+        const int64_t r = pix / W;
+        const int y = (int)(r % H);
+        const int64_t n = r / H;
+        Y[k] = c < Ca ? A[((n * (H / 2) + y / 2) * (W / 2) + x / 2) * Ca + c]
+                      : B[pix * Cb + (c - Ca)];
+    }
+}
+
+// dA[n][y][x][c] = (sum of dY over the 2x2 block, channel c) * (A > 0);
+// dB[p][c] = dY[p][Ca + c].
+__global__ void upcat_bwd_kernel(const bf16_t* __restrict__ dY, const bf16_t* __restrict__ A,
+                                 bf16_t* __restrict__ dA, bf16_t* __restrict__ dB, int N, int H,
+                                 int W, int Ca, int Cb)
+{
+    const int C = Ca + Cb, Ho = H / 2, Wo = W / 2;
+    const int64_t na = (int64_t)N * Ho * Wo * Ca, nb = (int64_t)N * H * W * Cb;
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < na + nb;
+         k += (int64_t)gridDim.x * blockDim.x) {
+        if (k < na) {
+            const int c = (int)(k % Ca);
+            int64_t r = k / Ca;
+            const int x = (int)(r % Wo);
+            r /= Wo;
+            const int y = (int)(r % Ho);
+            const int64_t n = r / Ho;
+            float g = 0.0f;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int yy = 2 * y + (q >> 1), xx = 2 * x + (q & 1);
+                g += bf2f(dY[((n * H + yy) * W + xx) * C + c]);
+            }
+            if (!(bf2f(A[k]) > 0.0f)) g = 0.0f;
+            dA[k] = f2bf(g);
+        } else {
+            const int64_t kb = k - na;
+            const int c = (int)(kb % Cb);
+            const int64_t pix = kb / Cb;
+            dB[kb] = dY[pix * C + Ca + c];
+        }
+    }
+}
+
+// ---- head, loss, optimizer ----------------------------------------------------------
+
+// out[p] = img[p] + hb + sum_c hw[c] * F[p][c]   (float result)
+__global__ void head_fwd_kernel(const bf16_t* __restrict__ F, const float* __restrict__ hw,
+                                const float* __restrict__ hb, const float* __restrict__ img,
+                                float* __restrict__ out, int64_t P, int C)
+{
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < P;
+         p += (int64_t)gridDim.x * blockDim.x) {
+        float s = hb[0];
+        for (int c = 0; c < C; ++c) s = fmaf(hw[c], bf2f(F[p * C + c]), s);
+        out[p] = img[p] + s;
+    }
+}
+
+// dF[p][c] = dout[p] * hw[c] * (F > 0); dhw[c] += sum_p dout[p] F[p][c]; dhb += sum dout.
+// Block of 256 threads; per-block sums in LDS, one atomic per block and channel.
+__global__ void __launch_bounds__(256)
+head_bwd_kernel(const bf16_t* __restrict__ F, const float* __restrict__ hw,
+                const float* __restrict__ dout, bf16_t* __restrict__ dF, float* __restrict__ dhw,
+                float* __restrict__ dhb, int64_t P, int C)
+{
+    __shared__ float red[33][256];
+    const int t = threadIdx.x;
+    float sw[32];
+    for (int c = 0; c < 32; ++c) sw[c] = 0.0f;
+    float sb = 0.0f;
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + t; p < P;
+         p += (int64_t)gridDim.x * blockDim.x) {
+        const float g = dout[p];
+        sb += g;
+        for (int c = 0; c < C; ++c) {
+            const float f = bf2f(F[p * C + c]);
+            sw[c] = fmaf(g, f, sw[c]);
+            dF[p * C + c] = f2bf(f > 0.0f ? g * hw[c] : 0.0f);
+        }
+    }
+    for (int c = 0; c < 32; ++c) red[c][t] = sw[c];
+    red[32][t] = sb;
+    __syncthreads();
+    if (t <= 32 && (t < C || t == 32)) {
+        float s = 0.0f;
+        for (int k = 0; k < 256; ++k) s += red[t][k];
+        if (t == 32) atomicAdd(dhb, s);
+        else atomicAdd(&dhw[t], s);
+    }
+}
+
+// d = out - target; grad = 2 d / P; partial sums of d^2 per block (FP64, fixed order).
+__global__ void __launch_bounds__(256)
+mse_kernel(const float* __restrict__ out, const float* __restrict__ target,
+           float* __restrict__ grad, double* __restrict__ partials, int64_t P)
+{
+    __shared__ double red[256];
+    double s = 0.0;
+    const float scale = 2.0f / (float)P;
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < P;
+         p += (int64_t)gridDim.x * blockDim.x) {
+        const float d = out[p] - target[p];
+        grad[p] = scale * d;
+        s += (double)d * (double)d;
+    }
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partials[blockIdx.x] = red[0];
+}
+
+__global__ void sum_partials_kernel(const double* __restrict__ partials, int n, double scale,
+                                    double* __restrict__ out)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        double s = 0.0;
+        for (int k = 0; k < n; ++k) s += partials[k];
+        out[0] = s * scale;
+    }
+}
+
+// Adam with bias correction on FP32 master weights; refreshes the BF16 copy (may be null).
+__global__ void adam_kernel(float* __restrict__ w, const float* __restrict__ g,
+                            float* __restrict__ m, float* __restrict__ v,
+                            bf16_t* __restrict__ wbf, int64_t n, float lr, float b1, float b2,
+                            float eps, float c1, float c2)
+{
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n;
+         k += (int64_t)gridDim.x * blockDim.x) {
+        const float gk = g[k];
+        const float mk = b1 * m[k] + (1.0f - b1) * gk;
+        const float vk = b2 * v[k] + (1.0f - b2) * gk * gk;
+        m[k] = mk;
+        v[k] = vk;
+        const float wk = w[k] - lr * (mk / c1) / (sqrtf(vk / c2) + eps);
+        w[k] = wk;
+        if (wbf) wbf[k] = f2bf(wk);
+    }
+}
+
+__global__ void to_bf16_kernel(const float* __restrict__ x, bf16_t* __restrict__ y, int64_t n)
+{
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n;
+         k += (int64_t)gridDim.x * blockDim.x)
+        y[k] = f2bf(x[k]);
+}
+
+unsigned grid_for(int64_t n, int block = 256)
+{
+    const int64_t g = (n + block - 1) / block;
+    return (unsigned)(g < 65536 ? (g > 0 ? g : 1) : 65536);
+}
+
+}  // namespace
+}  // namespace itomo
+
+using namespace itomo;
+
+extern "C" {
+
+int itomo_unet_conv3x3(const void* x, int n, int h, int w, int cin, const void* wt,
+                       const float* bias, int cout, const void* mask, int relu, void* y,
+                       void* stream)
+{
+    if (!x || !wt || !y || n < 1 || h < 1 || w < 1 || cin < 1 || cout < 1)
+        return set_error(ITOMO_EINVAL, "unet_conv3x3: bad argument");
+    const int64_t P = (int64_t)n * h * w;
+    if (P > ((int64_t)1 << 31) / 8 || (int64_t)9 * cin > (1 << 20))
+        return set_error(ITOMO_EINVAL, "unet_conv3x3: too large");
+    const bool vec = cin % 8 == 0 && ((uintptr_t)x % 16) == 0 && ((uintptr_t)wt % 16) == 0;
+    dim3 grid((unsigned)((P + 127) / 128), (unsigned)((cout + 31) / 32));
+    hipStream_t st = (hipStream_t)stream;
+    if (vec)
+        conv3x3_kernel<true><<<grid, 256, 0, st>>>((const bf16_t*)x, (const bf16_t*)wt, bias,
+                                                   (const bf16_t*)mask, (bf16_t*)y, n, h, w,
+                                                   cin, cout, relu);
+    else
+        conv3x3_kernel<false><<<grid, 256, 0, st>>>((const bf16_t*)x, (const bf16_t*)wt, bias,
+                                                    (const bf16_t*)mask, (bf16_t*)y, n, h, w,
+                                                    cin, cout, relu);
+    return check_hip(hipGetLastError(), "unet_conv3x3 launch");
+}
+
+int itomo_unet_flip(const void* wt, int cin, int cout, void* wf, void* stream)
+{
+    if (!wt || !wf || cin < 1 || cout < 1) return set_error(ITOMO_EINVAL, "unet_flip: bad argument");
+    const int64_t n = (int64_t)cout * 9 * cin;
+    flip_kernel<<<grid_for(n), 256, 0, (hipStream_t)stream>>>((const bf16_t*)wt, (bf16_t*)wf,
+                                                               cin, cout);
+    return check_hip(hipGetLastError(), "unet_flip launch");
+}
+
+int itomo_unet_wgrad(const void* x, const void* dy, int n, int h, int w, int cin, int cout,
+                     float* dw, float* db, void* stream)
+{
+    if (!x || !dy || !dw || n < 1 || h < 1 || w < 1 || cin < 1 || cout < 1 || cout > 1024)
+        return set_error(ITOMO_EINVAL, "unet_wgrad: bad argument");
+    const int64_t P = (int64_t)n * h * w;
+    hipStream_t st = (hipStream_t)stream;
+    const int ncol = 9 * cin;
+    const unsigned gx = (unsigned)((ncol + 63) / 64), gy = (unsigned)((cout + 63) / 64);
+    // split the pixels so that the grid holds ~2048 blocks, in multiples of 32 pixels
+    int64_t nz = 2048 / ((int64_t)gx * gy);
+    nz = nz < 1 ? 1 : (nz > 65535 ? 65535 : nz);
+    int64_t chunk = (P + nz - 1) / nz;
+    chunk = ((chunk + 31) / 32) * 32;
+    nz = (P + chunk - 1) / chunk;
+    wgrad_kernel<<<dim3(gx, gy, (unsigned)nz), 256, 0, st>>>((const bf16_t*)x, (const bf16_t*)dy,
+                                                             dw, n, h, w, cin, cout, chunk);
+    int rc = check_hip(hipGetLastError(), "unet_wgrad launch");
+    if (rc || !db) return rc;
+    const int64_t bchunk = 1024;
+    bias_grad_kernel<<<(unsigned)((P + bchunk - 1) / bchunk), cout, 0, st>>>(
+        (const bf16_t*)dy, db, P, cout, bchunk);
+    return check_hip(hipGetLastError(), "unet_bias_grad launch");
+}
+
+int itomo_unet_pool_fwd(const void* x, int n, int h, int w, int c, void* y, void* idx,
+                        void* stream)
+{
+    if (!x || !y || !idx || n < 1 || h < 2 || w < 2 || c < 1 || (h & 1) || (w & 1))
+        return set_error(ITOMO_EINVAL, "unet_pool_fwd: bad argument");
+    const int64_t no = (int64_t)n * (h / 2) * (w / 2) * c;
+    pool_fwd_kernel<<<grid_for(no), 256, 0, (hipStream_t)stream>>>(
+        (const bf16_t*)x, (bf16_t*)y, (unsigned char*)idx, n, h, w, c);
+    return check_hip(hipGetLastError(), "unet_pool_fwd launch");
+}
+
+int itomo_unet_pool_bwd(const void* dy, const void* idx, const void* skip, const void* xm,
+                        int n, int h, int w, int c, void* dx, void* stream)
+{
+    if (!dy || !idx || !xm || !dx || n < 1 || h < 2 || w < 2 || c < 1 || (h & 1) || (w & 1))
+        return set_error(ITOMO_EINVAL, "unet_pool_bwd: bad argument");
+    const int64_t ni = (int64_t)n * h * w * c;
+    pool_bwd_kernel<<<grid_for(ni), 256, 0, (hipStream_t)stream>>>(
+        (const bf16_t*)dy, (const unsigned char*)idx, (const bf16_t*)skip, (const bf16_t*)xm,
+        (bf16_t*)dx, n, h, w, c);
+    return check_hip(hipGetLastError(), "unet_pool_bwd launch");
+}
+
+int itomo_unet_upcat_fwd(const void* a, const void* b, int n, int h, int w, int ca, int cb,
+                         void* y, void* stream)
+{
+    if (!a || !b || !y || n < 1 || h < 2 || w < 2 || ca < 1 || cb < 1 || (h & 1) || (w & 1))
+        return set_error(ITOMO_EINVAL, "unet_upcat_fwd: bad argument");
+    const int64_t no = (int64_t)n * h * w * (ca + cb);
+    upcat_fwd_kernel<<<grid_for(no), 256, 0, (hipStream_t)stream>>>(
+        (const bf16_t*)a, (const bf16_t*)b, (bf16_t*)y, n, h, w, ca, cb);
+    return check_hip(hipGetLastError(), "unet_upcat_fwd launch");
+}
+
+int itomo_unet_upcat_bwd(const void* dy, const void* a, int n, int h, int w, int ca, int cb,
+                         void* da, void* db, void* stream)
+{
+    if (!dy || !a || !da || !db || n < 1 || h < 2 || w < 2 || ca < 1 || cb < 1 || (h & 1) ||
+        (w & 1))
+        return set_error(ITOMO_EINVAL, "unet_upcat_bwd: bad argument");
+    const int64_t nt = (int64_t)n * (h / 2) * (w / 2) * ca + (int64_t)n * h * w * cb;
+    upcat_bwd_kernel<<<grid_for(nt), 256, 0, (hipStream_t)stream>>>(
+        (const bf16_t*)dy, (const bf16_t*)a, (bf16_t*)da, (bf16_t*)db, n, h, w, ca, cb);
+    return check_hip(hipGetLastError(), "unet_upcat_bwd launch");
+}
+
+int itomo_unet_head_fwd(const void* f, const float* hw, const float* hb, const float* img,
+                        int64_t npixels, int c, float* out, void* stream)
+{
+    if (!f || !hw || !hb || !img || !out || npixels < 1 || c < 1 || c > 32)
+        return set_error(ITOMO_EINVAL, "unet_head_fwd: bad argument");
+    head_fwd_kernel<<<grid_for(npixels), 256, 0, (hipStream_t)stream>>>(
+        (const bf16_t*)f, hw, hb, img, out, npixels, c);
+    return check_hip(hipGetLastError(), "unet_head_fwd launch");
+}
+
+int itomo_unet_head_bwd(const void* f, const float* hw, const float* dout, int64_t npixels,
+                        int c, void* df, float* dhw, float* dhb, void* stream)
+{
+    if (!f || !hw || !dout || !df || !dhw || !dhb || npixels < 1 || c < 1 || c > 32)
+        return set_error(ITOMO_EINVAL, "unet_head_bwd: bad argument");
+    const unsigned g = grid_for(npixels) < 1024 ? grid_for(npixels) : 1024;
+    head_bwd_kernel<<<g, 256, 0, (hipStream_t)stream>>>((const bf16_t*)f, hw, dout,
+                                                         (bf16_t*)df, dhw, dhb, npixels, c);
+    return check_hip(hipGetLastError(), "unet_head_bwd launch");
+}
+
+int itomo_unet_mse(const float* out, const float* target, int64_t npixels, float* grad,
+                   double* loss, double* work, void* stream)
+{
+    if (!out || !target || !grad || !loss || !work || npixels < 1)
+        return set_error(ITOMO_EINVAL, "unet_mse: bad argument");
+    hipStream_t st = (hipStream_t)stream;
+    const unsigned g = grid_for(npixels) < ITOMO_UNET_MSE_WORK ? grid_for(npixels)
+                                                               : ITOMO_UNET_MSE_WORK;
+    mse_kernel<<<g, 256, 0, st>>>(out, target, grad, work, npixels);
+    int rc = check_hip(hipGetLastError(), "unet_mse launch");
+    if (rc) return rc;
+    sum_partials_kernel<<<1, 64, 0, st>>>(work, (int)g, 1.0 / (double)npixels, loss);
+    return check_hip(hipGetLastError(), "unet_mse sum launch");
+}
+
+int itomo_unet_adam(float* w, const float* g, float* m, float* v, void* wbf, int64_t n,
+                    float lr, float beta1, float beta2, float eps, int step, void* stream)
+{
+    if (!w || !g || !m || !v || n < 1 || step < 1)
+        return set_error(ITOMO_EINVAL, "unet_adam: bad argument");
+    const float c1 = 1.0f - powf(beta1, (float)step), c2 = 1.0f - powf(beta2, (float)step);
+    adam_kernel<<<grid_for(n), 256, 0, (hipStream_t)stream>>>(w, g, m, v, (bf16_t*)wbf, n, lr,
+                                                               beta1, beta2, eps, c1, c2);
+    return check_hip(hipGetLastError(), "unet_adam launch");
+}
+
+int itomo_unet_to_bf16(const float* x, int64_t n, void* y, void* stream)
+{
+    if (!x || !y || n < 1) return set_error(ITOMO_EINVAL, "unet_to_bf16: bad argument");
+    to_bf16_kernel<<<grid_for(n), 256, 0, (hipStream_t)stream>>>(x, (bf16_t*)y, n);
+    return check_hip(hipGetLastError(), "unet_to_bf16 launch");
+}
+
+}  // extern "C"

@@ -73,6 +73,22 @@ def ellipsoid_slices(npix, nslices, n_ellipsoids=60, seed=3000, dtype=np.float32
     return out
 
 
+def textured(npix, n_ellipses=10, seed=1, texture=0.15, corr=0.04, dtype=np.float32):
+    """Random ellipses plus a band-limited Gaussian random field texture on their
+    support: white noise low-passed by a Gaussian of correlation length ``corr`` (in
+    FOV units), normalised to unit standard deviation, scaled by ``texture`` and
+    added where the ellipses are nonzero; the result is clipped at 0."""
+    img = ellipses(npix, n_ellipses, seed, np.float64)
+    rng = np.random.default_rng(seed + 7919)
+    f = np.fft.fftfreq(npix) * npix / 2.0           # cycles per FOV half-width
+    k2 = f[:, None] ** 2 + f[None, :] ** 2
+    spec = np.exp(-2.0 * (np.pi * corr) ** 2 * k2)
+    grf = np.real(np.fft.ifft2(np.fft.fft2(rng.standard_normal((npix, npix))) * spec))
+    grf /= max(grf.std(), 1e-12)
+    img = img + texture * grf * (img > 0)
+    return np.maximum(img, 0.0).astype(dtype)
+
+
 def angles_deg(nang, start=0.0, end=180.0):
     """Equally spaced projection angles in degrees, right end excluded
     (reference algorithms/utils.py:104-106)."""

@@ -1,0 +1,230 @@
+"""The learned warm start's HIP kernels against the CPU float64 oracle (oracle/unet_ref.py).
+
+Bars:
+  * one convolution on BF16-exact inputs: the GPU accumulates in FP32 and rounds once to
+    BF16; the oracle accumulates in FP64 and rounds once.  They differ only where the two
+    sums fall on different sides of a BF16 rounding boundary: at most 1 BF16 ulp, on
+    under 1% of the outputs;
+  * weight / bias gradients (FP32 outputs from BF16 operands): relative L2 <= 1e-5;
+  * pooling, upsampling/concatenation and their gradients: exact (they only move or
+    compare BF16 values); the head in FP32: relative 1e-6;
+  * the whole network, forward loss and every parameter gradient, against the oracle
+    that rounds to BF16 at the same places: relative L2 <= 2e-2 per gradient (rounding
+    flips between FP32 and FP64 sums propagate through 15 layers); against the exact
+    FP64 network the BF16 error is reported and held to <= 1.5e-1.
+"""
+import ctypes
+
+import numpy as np
+import pytest
+import torch
+import torch.nn.functional as F
+
+from oracle import unet_ref as U
+from iterative_tomographic_reconstruction_algorithms_amd import _lib as L
+from iterative_tomographic_reconstruction_algorithms_amd.core import stream_handle
+from iterative_tomographic_reconstruction_algorithms_amd.unet import UNet
+
+pytestmark = pytest.mark.gpu
+
+
+def P(t):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None else None
+
+
+def rel(a, b):
+    a = a.double().cpu()
+    b = b.double().cpu()
+    return float((a - b).norm() / b.norm())
+
+
+def bf16_rand(shape, seed, scale=1.0, positive=False):
+    g = torch.Generator().manual_seed(seed)
+    x = torch.randn(shape, generator=g, dtype=torch.float64) * scale
+    if positive:
+        x = x.abs()
+    return x.to(torch.bfloat16)
+
+
+def ulp_diff(a_bf16, b_bf16):
+    """|difference| in BF16 ulps (same-sign values), via the integer bit patterns;
+    +0 and -0 count as equal."""
+    ai = a_bf16.view(torch.int16).to(torch.int32)
+    bi = b_bf16.view(torch.int16).to(torch.int32)
+    d = (ai - bi).abs()
+    both_zero = (a_bf16.float() == 0) & (b_bf16.float() == 0)
+    return torch.where(both_zero, torch.zeros_like(d), d)
+
+
+@pytest.mark.parametrize("n,h,w,cin,cout,relu", [(2, 16, 16, 32, 48, True),
+                                                  (1, 10, 14, 1, 32, True),
+                                                  (3, 8, 8, 96, 32, False),
+                                                  (1, 12, 12, 64, 64, True)])
+def test_conv3x3_matches_fp64(n, h, w, cin, cout, relu):
+    x = bf16_rand((n, h, w, cin), 1)
+    wt = bf16_rand((cout, 9, cin), 2, scale=(2.0 / (9 * cin)) ** 0.5)
+    bias = torch.randn(cout, generator=torch.Generator().manual_seed(3)) * 0.1
+    mask = bf16_rand((n, h, w, cout), 4)          # about half the entries > 0
+    for use_mask in (False, True):
+        y = torch.empty((n, h, w, cout), dtype=torch.bfloat16, device="cuda")
+        xd, wd, bd, md = x.cuda(), wt.cuda(), bias.float().cuda(), mask.cuda()
+        L.check(L.lib().itomo_unet_conv3x3(P(xd), n, h, w, cin, P(wd), P(bd), cout,
+                                           P(md) if use_mask else None, int(relu), P(y),
+                                           stream_handle()), "conv")
+        ref = F.conv2d(x.double().permute(0, 3, 1, 2), U.conv_weight(wt.double()),
+                       bias.double(), padding=1).permute(0, 2, 3, 1)
+        if relu:
+            ref = F.relu(ref)
+        if use_mask:
+            ref = torch.where(mask.double() > 0, ref, torch.zeros_like(ref))
+        ref16 = ref.to(torch.float32).to(torch.bfloat16)
+        got = y.cpu()
+        d = ulp_diff(got, ref16)
+        assert int(d.max()) <= 1
+        assert float((d > 0).double().mean()) < 0.01
+
+
+@pytest.mark.parametrize("n,h,w,cin,cout", [(2, 16, 16, 32, 64), (1, 12, 10, 1, 32),
+                                             (2, 8, 8, 96, 32), (1, 32, 32, 64, 128)])
+def test_wgrad_matches_fp64(n, h, w, cin, cout):
+    x = bf16_rand((n, h, w, cin), 5)
+    dy = bf16_rand((n, h, w, cout), 6)
+    dw = torch.zeros((cout, 9, cin), device="cuda")
+    db = torch.zeros(cout, device="cuda")
+    xd, dyd = x.cuda(), dy.cuda()   # keep both alive across the launch
+    L.check(L.lib().itomo_unet_wgrad(P(xd), P(dyd), n, h, w, cin, cout, P(dw), P(db),
+                                     stream_handle()), "wgrad")
+    ref = torch.nn.grad.conv2d_weight(x.double().permute(0, 3, 1, 2), (cout, cin, 3, 3),
+                                      dy.double().permute(0, 3, 1, 2), padding=1)
+    ref = ref.permute(0, 2, 3, 1).reshape(cout, 9, cin)
+    assert rel(dw, ref) <= 1e-5
+    assert rel(db, dy.double().sum(dim=(0, 1, 2))) <= 1e-5
+
+
+def test_flip_gives_input_gradient():
+    n, h, w, cin, cout = 2, 12, 12, 32, 64
+    dy = bf16_rand((n, h, w, cout), 7)
+    wt = bf16_rand((cout, 9, cin), 8, scale=0.1)
+    wf = torch.empty((cin, 9, cout), dtype=torch.bfloat16, device="cuda")
+    wtd, dyd = wt.cuda(), dy.cuda()
+    L.check(L.lib().itomo_unet_flip(P(wtd), cin, cout, P(wf), stream_handle()), "flip")
+    dx = torch.empty((n, h, w, cin), dtype=torch.bfloat16, device="cuda")
+    L.check(L.lib().itomo_unet_conv3x3(P(dyd), n, h, w, cout, P(wf), None, cin, None, 0,
+                                       P(dx), stream_handle()), "conv")
+    ref = torch.nn.grad.conv2d_input((n, cin, h, w), U.conv_weight(wt.double()),
+                                     dy.double().permute(0, 3, 1, 2), padding=1)
+    ref16 = ref.permute(0, 2, 3, 1).to(torch.float32).to(torch.bfloat16)
+    d = ulp_diff(dx.cpu(), ref16)
+    assert int(d.max()) <= 1 and float((d > 0).double().mean()) < 0.01
+
+
+def test_pool_upcat_head_exact():
+    n, h, w, c = 2, 8, 12, 32
+    e = bf16_rand((n, h, w, c), 9)
+    y = torch.empty((n, h // 2, w // 2, c), dtype=torch.bfloat16, device="cuda")
+    idx = torch.empty((n, h // 2, w // 2, c), dtype=torch.uint8, device="cuda")
+    ed = e.cuda()
+    L.check(L.lib().itomo_unet_pool_fwd(P(ed), n, h, w, c, P(y), P(idx), stream_handle()), "p")
+    pe, pi = F.max_pool2d(e.double().permute(0, 3, 1, 2), 2, return_indices=True)
+    assert torch.equal(y.cpu().double(), pe.permute(0, 2, 3, 1))
+    # backward: scatter + skip, masked by e > 0
+    dy = bf16_rand((n, h // 2, w // 2, c), 10)
+    skip = bf16_rand((n, h, w, c), 11)
+    dx = torch.empty((n, h, w, c), dtype=torch.bfloat16, device="cuda")
+    dyd, skd = dy.cuda(), skip.cuda()
+    L.check(L.lib().itomo_unet_pool_bwd(P(dyd), P(idx), P(skd), P(ed), n, h, w, c,
+                                        P(dx), stream_handle()), "pb")
+    ref = F.max_unpool2d(dy.double().permute(0, 3, 1, 2), pi, 2, output_size=(h, w))
+    ref = ref.permute(0, 2, 3, 1) + skip.double()
+    ref = torch.where(e.double() > 0, ref, torch.zeros_like(ref))
+    assert torch.equal(dx.cpu(), ref.to(torch.float32).to(torch.bfloat16))
+    # upsample + concat and its gradient
+    a = bf16_rand((n, h // 2, w // 2, 16), 12)
+    u = torch.empty((n, h, w, 16 + c), dtype=torch.bfloat16, device="cuda")
+    ad = a.cuda()
+    L.check(L.lib().itomo_unet_upcat_fwd(P(ad), P(ed), n, h, w, 16, c, P(u),
+                                         stream_handle()), "u")
+    ref_u = torch.cat([F.interpolate(a.double().permute(0, 3, 1, 2), scale_factor=2),
+                       e.double().permute(0, 3, 1, 2)], dim=1).permute(0, 2, 3, 1)
+    assert torch.equal(u.cpu().double(), ref_u)
+    du = bf16_rand((n, h, w, 16 + c), 13)
+    da = torch.empty_like(a, device="cuda")
+    db = torch.empty((n, h, w, c), dtype=torch.bfloat16, device="cuda")
+    dud = du.cuda()
+    L.check(L.lib().itomo_unet_upcat_bwd(P(dud), P(ad), n, h, w, 16, c, P(da), P(db),
+                                         stream_handle()), "ub")
+    ref_da = F.avg_pool2d(du.double().permute(0, 3, 1, 2)[:, :16], 2) * 4
+    ref_da = ref_da.permute(0, 2, 3, 1)
+    ref_da = torch.where(a.double() > 0, ref_da, torch.zeros_like(ref_da))
+    assert torch.equal(da.cpu(), ref_da.to(torch.float32).to(torch.bfloat16))
+    assert torch.equal(db.cpu(), du[..., 16:])
+    # head
+    f = bf16_rand((n, h, w, 32), 14, positive=True)
+    hw = torch.randn(32, generator=torch.Generator().manual_seed(15)).float()
+    hb = torch.tensor([0.25])
+    img = torch.rand((n, h, w))
+    out = torch.empty((n, h, w), device="cuda")
+    fd, hwd, hbd, imgd = f.cuda(), hw.cuda(), hb.cuda(), img.cuda()
+    L.check(L.lib().itomo_unet_head_fwd(P(fd), P(hwd), P(hbd), P(imgd), n * h * w, 32, P(out),
+                                        stream_handle()), "h")
+    ref_o = img.double() + 0.25 + (f.double() * hw.double()).sum(-1)
+    assert rel(out, ref_o) <= 1e-6
+
+
+def _rand_net(seed):
+    net = UNet(seed=seed)
+    g = torch.Generator().manual_seed(seed + 1)
+    # a non-trivial head and biases, so every path carries gradient
+    net.hw.copy_((torch.randn(32, generator=g) * 0.2).cuda())
+    net.hb.fill_(0.1)
+    for name in net.b:
+        net.b[name].copy_((torch.randn(net.b[name].numel(), generator=g) * 0.05).cuda())
+    net.refresh_bf16()
+    return net
+
+
+def _oracle_params(net):
+    p = {}
+    for name, t, tbf in net.params():
+        src = tbf if tbf is not None else t
+        p[name] = src.detach().double().cpu()
+    return p
+
+
+def test_network_gradients_match_oracle():
+    net = _rand_net(21)
+    g = torch.Generator().manual_seed(22)
+    img = torch.rand((2, 32, 32), generator=g)
+    target = img + 0.1 * torch.randn((2, 32, 32), generator=g)
+    net.zero_grad()
+    out = net.forward(img.cuda(), keep=True)
+    loss, grad = net.loss_and_grad(out, target.cuda())
+    net.backward(grad)
+    torch.cuda.synchronize()
+    params = _oracle_params(net)
+    ref_loss, ref_grads = U.loss_and_grads(params, img.double(), target.double(), bf16=True)
+    exact_loss, exact_grads = U.loss_and_grads(params, img.double(), target.double(), bf16=False)
+    assert abs(float(loss) - ref_loss) <= 1e-3 * ref_loss
+    worst_bf16, worst_exact = 0.0, 0.0
+    for name, t, _ in net.params():
+        gg = net.grad[name].cpu().double()
+        rb = rel(gg, ref_grads[name])
+        re = rel(gg, exact_grads[name])
+        worst_bf16, worst_exact = max(worst_bf16, rb), max(worst_exact, re)
+        assert rb <= 2e-2, (name, rb)
+        assert re <= 1.5e-1, (name, re)
+    print("worst gradient rel L2: vs BF16-faithful oracle %.2e, vs FP64 network %.2e"
+          % (worst_bf16, worst_exact))
+
+
+def test_training_reduces_loss():
+    net = UNet(seed=3)
+    g = torch.Generator().manual_seed(4)
+    # learn a fixed smoothing of random images (a task a U-Net fits quickly)
+    img = torch.rand((8, 32, 32), generator=g)
+    target = F.avg_pool2d(img[:, None], 3, stride=1, padding=1)[:, 0]
+    img, target = img.cuda(), target.cuda()
+    first = float(net.train_step(img, target, lr=2e-3))
+    for _ in range(150):
+        last = float(net.train_step(img, target, lr=2e-3))
+    assert last < 0.2 * first, (first, last)
