@@ -359,6 +359,95 @@ __device__ __forceinline__ float first_pixel(float Dlo, float K, int nh, int npi
 
 constexpr int kFwdLines = 4;   // lines whose gathers are in flight together
 
+// image[idx .. idx+3] of one slice through the buffer descriptor (out of range: 0).
+__device__ __forceinline__ float4 load_px4(__amdgpu_buffer_rsrc_t rs, int idx)
+{
+    typedef int int4v __attribute__((ext_vector_type(4)));
+    const int4v v = __builtin_amdgcn_raw_buffer_load_b128(rs, (int)((unsigned)idx * 4u), 0, 0);
+    return make_float4(__int_as_float(v[0]), __int_as_float(v[1]), __int_as_float(v[2]),
+                       __int_as_float(v[3]));
+}
+
+// Forward projection of one sweep view whose bins are narrower than 1.75 pixels (at
+// most 3 candidate pixels per bin and line), for the B bins [bx*B, +B) of a block.
+// Lines are processed in chunks of FL: for each line of a chunk the block first copies
+// the window of 2B pixels its bins can touch into LDS with 16-byte loads (one contiguous
+// row segment per line; the window starts at the first bin's estimated first pixel
+// minus 2, rounded down to a multiple of 4), then every thread reads its candidates
+// from LDS.  The block's bins span at most (B-1)*1.75 + 1 pixels, so the window always
+// holds them.  Lines run over the union of the block's per-bin line ranges; lines that
+// miss a bin add nothing to it.  Every thread of the block takes part (inactive
+// threads, past the last bin, compute a copy of the last bin and store nothing).
+template <int ARITH, int B>
+__device__ __forceinline__ float fwd_staged(const DdView& g, const float* __restrict__ ls,
+                                            int npix, int nh, __amdgpu_buffer_rsrc_t rs,
+                                            int bx, int d, float acc)
+{
+    constexpr int W = 2 * B;
+    constexpr int FL = 8;  // 8 x 2B floats = 64B bytes per block: 1 KB per wave, full occupancy
+    __shared__ __attribute__((aligned(16))) float sImg[FL][W];
+    __shared__ int sQ0[FL];
+    const int tid = threadIdx.x;
+    const int dk = min(d, npix - 1);
+    const float Dlo = key_D<ARITH>(g, dk - nh);
+    const float Dhi = key_D<ARITH>(g, dk + 1 - nh);
+    const bool last_bin = dk == npix - 1;
+    const float xNm1 = (float)(npix - 1 - nh);
+    const int d0 = bx * B, d1 = min(d0 + B - 1, npix - 1);
+    const float D0lo = key_D<ARITH>(g, d0 - nh);
+    const float D1hi = key_D<ARITH>(g, d1 + 1 - nh);
+    const float x0 = (float)(-nh), xN = (float)(npix - nh);
+    double ua = ((double)x0 - (double)D1hi) / (double)g.t;
+    double ub = ((double)xN - (double)D0lo) / (double)g.t;
+    if (ua > ub) { const double tmp = ua; ua = ub; ub = tmp; }
+    ua = fmin(fmax(ua + nh - 0.5, -4.0), (double)npix + 4.0);
+    ub = fmin(fmax(ub + nh - 0.5, -4.0), (double)npix + 4.0);
+    const int i_lo = max(0, (int)floor(ua) - 2);
+    const int i_hi = min(npix - 1, (int)ceil(ub) + 2);
+    float u = line_coord(i_lo, nh);
+    for (int c0 = i_lo; c0 <= i_hi; c0 += FL) {
+        const int ncl = min(FL, i_hi - c0 + 1);
+        __syncthreads();  // the previous chunk is consumed
+        if (tid < ncl) {
+            const float K = key_K<ARITH>(g, line_coord(c0 + tid, nh));
+            const float est = fminf(fmaxf(floorf(__fadd_rn(D0lo, K)), (float)(-nh - W)),
+                                    (float)(npix - nh + W));
+            sQ0[tid] = ((int)est + nh - 2) & ~3;
+        }
+        __syncthreads();
+        for (int k = tid; k < ncl * (W / 4); k += B) {
+            const int l = k / (W / 4), j4 = k - l * (W / 4);
+            const int q = sQ0[l] + 4 * j4;
+            // a window that starts left of the row reads zeros there (negative index ->
+            // out of range); past the row's end it reads the next row: never weighted
+            const int idx = (c0 + l) * npix + q;
+            float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (q >= 0 && idx + 3 < npix * npix) {
+                v = load_px4(rs, idx);
+            } else if (q >= 0) {  // straddles the end of the slice: per-element checks
+                v.x = load_px(rs, idx);
+                v.y = load_px(rs, idx + 1);
+                v.z = load_px(rs, idx + 2);
+                v.w = load_px(rs, idx + 3);
+            }
+            *(float4*)&sImg[l][4 * j4] = v;
+        }
+        __syncthreads();
+        for (int l = 0; l < ncl; ++l, u = __fadd_rn(u, 1.0f)) {
+            const float K = key_K<ARITH>(g, u);
+            const float x = first_pixel(Dlo, K, nh, npix);
+            const int off = min(max((int)x + nh - sQ0[l], 0), W - 3);
+            float Dhc = Dhi;
+            if (last_bin || Dhi > __fsub_rn(xNm1, K)) Dhc = fminf(Dhi, ls[c0 + l]);
+            float xk = x, Plo = __fsub_rn(x, K);
+#pragma unroll
+            for (int k = 0; k < 3; ++k)
+                acc = cand_step<ARITH>(k == 0, xk, Plo, K, Dlo, Dhc, sImg[l][off + k], acc);
+        }
+    }
+    return acc;
+}
+
 // Forward projection: one thread per (bin, view blockIdx.y, slice blockIdx.z).
 // Lines are image rows for row sweeps and image columns for column sweeps; column
 // sweeps (and row sums) read the transposed image so that the lanes of a wave, which
@@ -383,7 +472,15 @@ __device__ __forceinline__ void dd_fwd_block(const DdView* __restrict__ geom,
     const int nh = (int)(npix * 0.5);
     double p0 = 0.0;
 
-    if (active) {
+    const bool sweep = g.kind == DD_SWEEP_ROWS || g.kind == DD_SWEEP_COLS;
+    if (!ONES && sweep && g.ncand <= 3) {
+        // block-uniform: every thread joins the LDS-staged path
+        const __amdgpu_buffer_rsrc_t rs =
+            image_rsrc((g.kind == DD_SWEEP_ROWS) ? image : imageT, npix);
+        float acc = (EPI == EPI_STORE && accumulate && active) ? out[soff + o] : 0.0f;
+        acc = fwd_staged<ARITH, B>(g, lstar + (int64_t)view * npix, npix, nh, rs, bx, d, acc);
+        if (active) out[soff + o] = epi_forward<EPI>(acc, o, soff, z, e, &p0);
+    } else if (active) {
         float acc = (EPI == EPI_STORE && accumulate) ? out[soff + o] : 0.0f;
         if (g.kind == DD_ROWSUM) {
             // sum_j img[d][j], in j order, read as column d of the transpose

@@ -28,7 +28,7 @@ from .phantom import angles_deg, ellipses, textured
 from .unet import UNet
 
 __all__ = ["dd_plan", "fbp", "sirt", "residual", "make_pairs", "train", "iterations_to",
-           "compare_starts"]
+           "compare_starts", "save_weights", "load_weights"]
 
 
 def dd_plan(npix, nang):
@@ -200,6 +200,19 @@ def compare_starts(plan, net, b, plain_iters, max_iters):
     }
 
 
+def save_weights(net, path):
+    """FP32 master weights as an .npz of plain arrays (no pickled objects)."""
+    np.savez(path, **{k: v.numpy().astype(np.float32) for k, v in net.state_dict().items()})
+
+
+def load_weights(path, seed=0):
+    """A UNet with the weights of an .npz written by save_weights."""
+    net = UNet(seed=seed)
+    with np.load(path, allow_pickle=False) as z:
+        net.load_state_dict({k: torch.from_numpy(z[k]) for k in z.files})
+    return net
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description="train and evaluate the SIRT warm start")
     ap.add_argument("--npix", type=int, default=256)
@@ -213,12 +226,17 @@ def main(argv=None):
     ap.add_argument("--noiseless", action="store_true")
     ap.add_argument("--test", type=int, default=32)
     ap.add_argument("--max-iters", type=int, default=2000)
+    ap.add_argument("--ellipses", type=int, default=10)
+    ap.add_argument("--texture", type=float, default=0.15)
+    ap.add_argument("--chunk", type=int, default=64, help="slices per SIRT launch")
     ap.add_argument("--save", default="")
     args = ap.parse_args(argv)
     plan = dd_plan(args.npix, args.nang)
     noise = None if args.noiseless else (1e3, 1e5)
     t0 = time.perf_counter()
-    xin, xout, _, _ = make_pairs(plan, args.pairs, args.sirt_iters, seed=1, noise=noise)
+    xin, xout, _, _ = make_pairs(plan, args.pairs, args.sirt_iters, seed=1, noise=noise,
+                                 n_ellipses=args.ellipses, texture=args.texture,
+                                 chunk=args.chunk)
     torch.cuda.synchronize()
     t_data = time.perf_counter() - t0
     net = UNet(seed=0)
@@ -228,13 +246,15 @@ def main(argv=None):
     torch.cuda.synchronize()
     t_train = time.perf_counter() - t0
     del xin, xout
-    _, _, b_test, _ = make_pairs(plan, args.test, 1, seed=2, noise=noise)
+    _, _, b_test, _ = make_pairs(plan, args.test, 1, seed=2, noise=noise,
+                                 n_ellipses=args.ellipses, texture=args.texture,
+                                 chunk=args.chunk)
     res = compare_starts(plan, net, b_test, args.sirt_iters, args.max_iters)
     res.update({"pairs": args.pairs, "train_steps": args.steps, "seconds_data": t_data,
                 "seconds_train": t_train, "loss_first": float(losses[:10].mean()),
                 "loss_last": float(losses[-10:].mean()), "parameters": net.num_parameters()})
     if args.save:
-        np.savez(args.save, **{k: v.numpy() for k, v in net.state_dict().items()})
+        save_weights(net, args.save)
     print(json.dumps(res), flush=True)
     return res
 
