@@ -37,8 +37,8 @@ def sources():
 
 # extra flags per translation unit
 PER_FILE = {
-    "dd.hip": ["-fno-honor-nans"],
-    "batch.hip": ["-fno-honor-nans"],
+    "dd.hip": ["-fno-honor-nans", "-falign-loops=64"],
+    "batch.hip": ["-fno-honor-nans", "-falign-loops=64"],
 }
 
 

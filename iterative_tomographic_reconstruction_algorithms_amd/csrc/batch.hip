@@ -211,7 +211,7 @@ int itomo_batch_create(itomo_batch** out, int nmembers, const itomo_plan* const*
         bm.C = C[m];
         bm.work = work[m];
         // the batch launches the default shapes whatever the plan's own choice
-        bm.bps_f = (int)dd_fwd_blocks(p->npix, p->nang, kDdFwdBlock);
+        bm.bps_f = (int)dd_fwd_slots(p->npix, p->nang, kDdFwdBlock);
         bm.bps_b = (int)dd_back_blocks(p->npix, kDdBackTile);
         B->h_members.push_back(bm);
         B->plans.push_back(p);

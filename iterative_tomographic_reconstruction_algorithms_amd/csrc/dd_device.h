@@ -227,6 +227,23 @@ __device__ __forceinline__ void dd_back_tile(const DdView* __restrict__ geom,
                             ? out[(int64_t)r * npix + q]
                             : 0.0f;
         }
+    // With small micro-tiles the epilogue inputs (current iterate and weight) are
+    // loaded up front, so their latency hides behind the view loop.
+    constexpr bool PRE = P <= 2 && (epi_updates_x(EPI) || epi_back_uses_w(EPI));
+    float pre_x[P][P], pre_w[P][P];
+#pragma unroll
+    for (int i = 0; i < P; ++i)
+#pragma unroll
+        for (int j = 0; j < P; ++j) {
+            pre_x[i][j] = 0.0f;
+            pre_w[i][j] = 0.0f;
+            const int r = rr + i, q = cc + j;
+            if (PRE && r < npix && q < npix) {
+                const int64_t o = (int64_t)r * npix + q;
+                if (epi_updates_x(EPI)) pre_x[i][j] = out[o];
+                if (epi_back_uses_w(EPI)) pre_w[i][j] = e.w[(int64_t)z * e.w_stride + o];
+            }
+        }
 
     for (int v0 = 0; v0 < nang; v0 += kBackViews) {
         const int nv = min(kBackViews, nang - v0);
@@ -296,7 +313,9 @@ __device__ __forceinline__ void dd_back_tile(const DdView* __restrict__ geom,
             if (r < npix && q < npix) {
                 const int64_t o = (int64_t)r * npix + q;
                 double q1 = 0.0, q2 = 0.0;
-                if (epi_updates_x(EPI))
+                if (PRE)
+                    out[o] = epi_back_v<EPI>(acc[i][j], pre_x[i][j], pre_w[i][j], e, &q1, &q2);
+                else if (epi_updates_x(EPI))
                     out[o] = epi_back<EPI>(acc[i][j], out[o], o, z, e, &q1, &q2);
                 else
                     out[o] = epi_back<EPI>(acc[i][j], 0.0f, o, z, e, &q1, &q2);
@@ -474,12 +493,14 @@ __device__ __forceinline__ void dd_fwd_block(const DdView* __restrict__ geom,
 
     const bool sweep = g.kind == DD_SWEEP_ROWS || g.kind == DD_SWEEP_COLS;
     if (!ONES && sweep && g.ncand <= 3) {
-        // block-uniform: every thread joins the LDS-staged path
+        // block-uniform: every thread joins the LDS-staged path; the epilogue inputs
+        // are loaded first so that their latency hides behind the sweep
+        const EpiFwdIn pre = active ? epi_fwd_load<EPI>(o, soff, z, e) : EpiFwdIn{0.0f, 0.0f};
         const __amdgpu_buffer_rsrc_t rs =
             image_rsrc((g.kind == DD_SWEEP_ROWS) ? image : imageT, npix);
         float acc = (EPI == EPI_STORE && accumulate && active) ? out[soff + o] : 0.0f;
         acc = fwd_staged<ARITH, B>(g, lstar + (int64_t)view * npix, npix, nh, rs, bx, d, acc);
-        if (active) out[soff + o] = epi_forward<EPI>(acc, o, soff, z, e, &p0);
+        if (active) out[soff + o] = epi_forward<EPI>(acc, pre, &p0);
     } else if (active) {
         float acc = (EPI == EPI_STORE && accumulate) ? out[soff + o] : 0.0f;
         if (g.kind == DD_ROWSUM) {
@@ -584,8 +605,9 @@ __device__ __forceinline__ void dd_fwd_block(const DdView* __restrict__ geom,
         out[soff + o] = epi_forward<EPI>(acc, o, soff, z, e, &p0);
     }
     if (epi_has_partials(EPI) && e.partials)
-        block_partials3(p0, 0.0, 0.0,
-                        e.partials + ((int64_t)z * e.blocks_per_slice + blk) * 3);
+        wave_partials3(p0, 0.0, 0.0,
+                       e.partials + ((int64_t)z * e.blocks_per_slice +
+                                     (int64_t)blk * (B / 64) + (threadIdx.x >> 6)) * 3);
 }
 
 // out[c][r] = in[r][c] for one 32x32 tile of one npix x npix slice, through LDS

@@ -28,6 +28,23 @@ struct EpiArgs {
 
 // Block-wide deterministic sum of three doubles; thread 0 of the block writes them.
 // Requires blockDim.x*blockDim.y*blockDim.z a multiple of 64 and at most 1024.
+// Per-wave partial sums of a 1-D block: lane 0 of each wave writes its wave's three
+// sums to dst[0..2].  No barrier, so waves that finish early leave the compute unit at
+// once (the forward's lanes walk line ranges of different lengths).
+__device__ __forceinline__ void wave_partials3(double a0, double a1, double a2, double* dst)
+{
+    for (int off = 32; off > 0; off >>= 1) {
+        a0 += __shfl_down(a0, off, 64);
+        a1 += __shfl_down(a1, off, 64);
+        a2 += __shfl_down(a2, off, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        dst[0] = a0;
+        dst[1] = a1;
+        dst[2] = a2;
+    }
+}
+
 __device__ __forceinline__ void block_partials3(double a0, double a1, double a2,
                                                 double* dst)
 {
@@ -59,24 +76,38 @@ __device__ __forceinline__ void block_partials3(double a0, double a1, double a2,
     }
 }
 
-// Forward-side epilogue for output element index `o` (within one slice) of slice z.
-// Returns the value to store; writes the residual term to *p0.
+// Inputs of the forward-side epilogue of one output element, loaded before the
+// projection sum so that their latency overlaps it.
+struct EpiFwdIn {
+    float b, w;
+};
+
 template <int EPI>
-__device__ __forceinline__ float epi_forward(float acc, int64_t o, int64_t slice_off, int z,
-                                             const EpiArgs& e, double* p0)
+__device__ __forceinline__ EpiFwdIn epi_fwd_load(int64_t o, int64_t slice_off, int z,
+                                                 const EpiArgs& e)
+{
+    EpiFwdIn in{0.0f, 0.0f};
+    if (EPI == EPI_SPS_FWD || EPI == EPI_SIRT_FWD || EPI == EPI_MLEM_FWD)
+        in.b = e.b[slice_off + o];
+    if (EPI == EPI_SPS_FWD || EPI == EPI_SIRT_FWD) in.w = e.w[(int64_t)z * e.w_stride + o];
+    return in;
+}
+
+// Forward-side epilogue for one output element with its loaded inputs.  Returns the
+// value to store; writes the residual term to *p0.
+template <int EPI>
+__device__ __forceinline__ float epi_forward(float acc, const EpiFwdIn& in, double* p0)
 {
     if (EPI == EPI_SPS_FWD) {
-        const float bv = e.b[slice_off + o];
-        const float dif = __fmul_rn(__fsub_rn(acc, bv), e.w[(int64_t)z * e.w_stride + o]);
+        const float dif = __fmul_rn(__fsub_rn(acc, in.b), in.w);
         *p0 = (double)dif * (double)dif;
         return dif;
     } else if (EPI == EPI_SIRT_FWD) {
-        const float bv = e.b[slice_off + o];
-        const float dif = __fsub_rn(bv, acc);
+        const float dif = __fsub_rn(in.b, acc);
         *p0 = (double)dif * (double)dif;
-        return __fmul_rn(e.w[(int64_t)z * e.w_stride + o], dif);
+        return __fmul_rn(in.w, dif);
     } else if (EPI == EPI_MLEM_FWD) {
-        const float bv = e.b[slice_off + o];
+        const float bv = in.b;
         float ax = acc;
         if (ax == 0.0f) ax = __fadd_rn(ax, 2.220446049250313e-16f);
         const double dd = (double)ax - (double)bv;
@@ -90,28 +121,41 @@ __device__ __forceinline__ float epi_forward(float acc, int64_t o, int64_t slice
     return acc;
 }
 
-// Back-side epilogue for pixel index `o` of slice z: `xo` is the current iterate value.
+// Forward-side epilogue for output element index `o` (within one slice) of slice z.
 template <int EPI>
-__device__ __forceinline__ float epi_back(float acc, float xo, int64_t o, int z,
-                                          const EpiArgs& e, double* p1, double* p2)
+__device__ __forceinline__ float epi_forward(float acc, int64_t o, int64_t slice_off, int z,
+                                             const EpiArgs& e, double* p0)
 {
-    const int64_t wo = (int64_t)z * e.w_stride + o;
+    return epi_forward<EPI>(acc, epi_fwd_load<EPI>(o, slice_off, z, e), p0);
+}
+
+constexpr bool epi_back_uses_w(int epi)
+{
+    return epi == EPI_SPS_BACK || epi == EPI_SIRT_BACK || epi == EPI_MLEM_BACK;
+}
+
+// Back-side epilogue of one pixel: `xo` is the current iterate value and `w` the
+// pixel's weight (C, 1/A^T 1 or the SPS denominator), both already loaded.
+template <int EPI>
+__device__ __forceinline__ float epi_back_v(float acc, float xo, float w, const EpiArgs& e,
+                                            double* p1, double* p2)
+{
     if (EPI == EPI_SPS_BACK) {
-        const double q = (double)acc / ((double)e.w[wo] + 2.220446049250313e-16);
+        const double q = (double)acc / ((double)w + 2.220446049250313e-16);
         const float xn = (float)((double)xo - q);
         const double dd = (double)xn - (double)xo;
         *p1 = dd * dd;
         *p2 = (double)xo * (double)xo;
         return xn;
     } else if (EPI == EPI_SIRT_BACK) {
-        float xn = __fadd_rn(xo, __fmul_rn(e.w[wo], acc));
+        float xn = __fadd_rn(xo, __fmul_rn(w, acc));
         if (e.pc_beta >= 0.0f && xn < 0.0f) xn = __fmul_rn(xn, e.pc_beta);
         const double dd = (double)xn - (double)xo;
         *p1 = dd * dd;
         *p2 = (double)xo * (double)xo;
         return xn;
     } else if (EPI == EPI_MLEM_BACK) {
-        const float xn = __fmul_rn(xo, __fmul_rn(e.w[wo], acc));
+        const float xn = __fmul_rn(xo, __fmul_rn(w, acc));
         const double dd = (double)xn - (double)xo;
         *p1 = dd * dd;
         *p2 = (double)xo * (double)xo;
@@ -122,6 +166,15 @@ __device__ __forceinline__ float epi_back(float acc, float xo, int64_t o, int z,
         return __fdiv_rn(1.0f, v);
     }
     return acc;
+}
+
+// Back-side epilogue for pixel index `o` of slice z: `xo` is the current iterate value.
+template <int EPI>
+__device__ __forceinline__ float epi_back(float acc, float xo, int64_t o, int z,
+                                          const EpiArgs& e, double* p1, double* p2)
+{
+    const float w = epi_back_uses_w(EPI) ? e.w[(int64_t)z * e.w_stride + o] : 0.0f;
+    return epi_back_v<EPI>(acc, xo, w, e, p1, p2);
 }
 
 constexpr bool epi_has_partials(int epi)
@@ -142,7 +195,7 @@ int finalize_partials(const double* partials, int blocks_per_slice, int nslices,
 int elementwise_back_epilogue(int epi, const float* acc_img, float* x, const EpiArgs& e,
                               int64_t npix2, int nslices, hipStream_t st);
 int elementwise_fwd_epilogue(int epi, float* sino_io, const EpiArgs& e, int64_t nsino,
-                             int nslices, hipStream_t st);
+                             int nslices, hipStream_t st, int* blocks_per_slice);
 
 // projector launchers with epilogues (dd.hip / rd.hip)
 int dd_forward_epi(const Plan& p, int epi, int ones_input, const float* image, float* out,

@@ -102,6 +102,45 @@ int elementwise_back_epilogue(int epi, const float* acc_img, float* x, const Epi
     return check_hip(hipGetLastError(), "back epilogue launch");
 }
 
+// Elementwise forward epilogue over nslices sinograms of n entries, in place.
+template <int EPI>
+__global__ void __launch_bounds__(256)
+fwd_epi_kernel(float* __restrict__ io, EpiArgs e, int64_t n)
+{
+    const int z = blockIdx.y;
+    const int64_t soff = (int64_t)z * n;
+    double p0 = 0.0;
+    const int64_t base = (int64_t)blockIdx.x * kEpiChunk;
+    for (int k = 0; k < kEpiChunk / 256; ++k) {
+        const int64_t o = base + (int64_t)k * 256 + threadIdx.x;
+        if (o < n) {
+            double q0 = 0.0;
+            const EpiFwdIn in = epi_fwd_load<EPI>(o, soff, z, e);
+            io[soff + o] = epi_forward<EPI>(io[soff + o], in, &q0);
+            p0 += q0;
+        }
+    }
+    if (epi_has_partials(EPI) && e.partials)
+        block_partials3(p0, 0.0, 0.0,
+                        e.partials + ((int64_t)z * e.blocks_per_slice + blockIdx.x) * 3);
+}
+
+int elementwise_fwd_epilogue(int epi, float* sino_io, const EpiArgs& e, int64_t n,
+                             int nslices, hipStream_t st, int* blocks_per_slice)
+{
+    EpiArgs ee = e;
+    ee.blocks_per_slice = (int)((n + kEpiChunk - 1) / kEpiChunk);
+    if (blocks_per_slice) *blocks_per_slice = ee.blocks_per_slice;
+    dim3 grid(ee.blocks_per_slice, nslices);
+    switch (epi) {
+    case EPI_SIRT_FWD: fwd_epi_kernel<EPI_SIRT_FWD><<<grid, 256, 0, st>>>(sino_io, ee, n); break;
+    case EPI_SPS_FWD: fwd_epi_kernel<EPI_SPS_FWD><<<grid, 256, 0, st>>>(sino_io, ee, n); break;
+    case EPI_MLEM_FWD: fwd_epi_kernel<EPI_MLEM_FWD><<<grid, 256, 0, st>>>(sino_io, ee, n); break;
+    default: return set_error(ITOMO_EINVAL, "elementwise forward epilogue %d unsupported", epi);
+    }
+    return check_hip(hipGetLastError(), "forward epilogue launch");
+}
+
 // ---- FP64 dot product ---------------------------------------------------------------
 constexpr int kDotBlocks = 1024;
 

@@ -87,6 +87,13 @@ inline size_t dd_fwd_blocks(int npix, int nang, int bins)
     return (size_t)((npix + bins - 1) / bins) * nang;
 }
 
+// FP64 partial-sum slots of one forward slice: one per wave (the forward reduces per
+// wave, not per block).
+inline size_t dd_fwd_slots(int npix, int nang, int bins)
+{
+    return dd_fwd_blocks(npix, nang, bins) * (size_t)(bins / 64);
+}
+
 inline size_t dd_back_blocks(int npix, int tile)
 {
     const size_t t = (size_t)((npix + tile - 1) / tile);
@@ -96,7 +103,7 @@ inline size_t dd_back_blocks(int npix, int tile)
 inline size_t fwd_blocks_per_slice(const Plan& p)
 {
     return p.projector == ITOMO_PROJ_DD
-               ? dd_fwd_blocks(p.npix, p.nang, kFwdBins[p.fwd_var])
+               ? dd_fwd_slots(p.npix, p.nang, kFwdBins[p.fwd_var])
                : (size_t)((p.npix + kRdBlock - 1) / kRdBlock) * p.nang;
 }
 

@@ -1,6 +1,7 @@
 """Batched persistent SIRT over mixed geometries and hipGraph replay: both must equal the
 per-plan eager path bit for bit (the work schedule never changes an output element's
-accumulation order).  Graph replays are interleaved with eager work and every replay is
+accumulation order); the FP64 monitors agree to 1e-12 (their partial sums are grouped
+differently).  Graph replays are interleaved with eager work and every replay is
 checked against the eager result."""
 import numpy as np
 import pytest
@@ -44,7 +45,9 @@ def test_batch_of_mixed_geometries_equals_per_plan():
     for m in members:
         x_ref, st_ref = eager_sirt(m.plan, m.b, 4)
         assert torch.equal(m.x, x_ref)
-        assert torch.equal(m.stats, st_ref)
+        # FP64 monitors: the batch reduces the residual per wave inside the sweep, the
+        # per-plan path per block of its epilogue pass; same terms, other grouping
+        torch.testing.assert_close(m.stats, st_ref, rtol=1e-12, atol=0)
 
 
 def test_graph_replay_equals_eager():
