@@ -37,6 +37,21 @@ def alg_bytes_back(n, v):
     return 4.0 * n * n * v + 4.0 * v * n + 3 * 4.0 * n * n
 
 
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC summary
+    (profiles/traffic_dd_sirt_2048x1800.json: FETCH_SIZE and WRITE_SIZE per dispatch,
+    each scaled by the factor a 256 MiB copy kernel gives), or None when the summary
+    does not cover this kernel."""
+    path = os.path.join(ROOT, "profiles", "traffic_dd_sirt_2048x1800.json")
+    try:
+        with open(path) as f:
+            table = json.load(f)
+    except (OSError, ValueError):
+        return None
+    v = table.get("bytes_per_launch", {}).get(kernel)
+    return None if v is None else float(v)
+
+
 def cpu_baseline(img, c_angles, budget_views=20):
     """Reference C (radon_dd.c built with its own flags into oracle/_ref) timed on a
     bounded sample: forward + back of `budget_views` evenly spaced views at full N,
@@ -258,13 +273,16 @@ def main():
     gups = 2.0 * N * N * V * K / secs / 1e9
     resid_rel = float(stats[0].sqrt()) / bnorm
 
+    fwd_name = "dd_fwd_kernel<FAST,STORE,%d> + fwd_epi_kernel<SIRT_FWD>" % (
+        [256, 128, 64][shapes["fwd"]])
+    back_name = "dd_back_kernel<FAST,SIRT_BACK,%d,%d>" % tuple(
+        [(64, 4), (32, 2), (32, 4)][shapes["back"]])
     if back_ms >= fwd_ms:
-        dom, dom_ms, dom_bytes = "dd_back_kernel<FAST,SIRT_BACK,%d,%d>" % tuple(
-            [(64, 4), (32, 2), (32, 4)][shapes["back"]]), back_ms, alg_bytes_back(N, V)
+        dom, dom_ms, dom_bytes = back_name, back_ms, alg_bytes_back(N, V)
     else:
-        dom, dom_ms, dom_bytes = "dd_fwd_kernel<FAST,SIRT_FWD,%d>" % (
-            [256, 128, 64][shapes["fwd"]]), fwd_ms, alg_bytes_forward(N, V)
+        dom, dom_ms, dom_bytes = fwd_name, fwd_ms, alg_bytes_forward(N, V)
     achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
+    pair_gbs = (alg_bytes_forward(N, V) + alg_bytes_back(N, V)) / ((fwd_ms + back_ms) * 1e-3) / 1e9
     line = {
         "metric": "SIRT Gupdates/s & s-to-residual 1e-3 (plain vs warm-start) @2048²×1800 views",
         "value": round(gups, 3),
@@ -285,7 +303,11 @@ def main():
                               "fwd_ms": shapes["fwd_ms"], "back_ms": shapes["back_ms"]}},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": None},
+                     "traffic": measured_traffic(dom),
+                     "pair": {"what": "forward + back of one SIRT step",
+                              "alg_bytes": alg_bytes_forward(N, V) + alg_bytes_back(N, V),
+                              "achieved": round(pair_gbs, 1),
+                              "frac": round(pair_gbs / HBM_PEAK_GBS, 4)}},
         "kernels_ms": {"sirt_forward": round(fwd_ms, 4), "sirt_back": round(back_ms, 4)},
         "residual_after_timed_steps": resid_rel,
     }
