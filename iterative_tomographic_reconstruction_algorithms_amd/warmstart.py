@@ -79,9 +79,11 @@ def _noisy(plan, x_true, i0, mu_max, gen):
 
 
 def make_pairs(plan, count, sirt_iters, seed=0, noise=(1e3, 1e5), mu_max=3.0,
-               n_ellipses=10, texture=0.15, chunk=64):
+               n_ellipses=10, texture=0.15, chunk=64, input_iters=0):
     """(inputs, targets, sinograms, phantoms), each (count, ...) float32 on the GPU.
-    ``noise`` = (I0_lo, I0_hi) or None for noiseless data."""
+    ``noise`` = (I0_lo, I0_hi) or None for noiseless data.  The input is the FBP
+    (``input_iters`` = 0) or the plain SIRT iterate after ``input_iters`` iterations,
+    which SIRT then continues to the target."""
     dev = device()
     gen = torch.Generator(device=dev).manual_seed(seed)
     rng = np.random.default_rng(seed)
@@ -99,8 +101,13 @@ def make_pairs(plan, count, sirt_iters, seed=0, noise=(1e3, 1e5), mu_max=3.0,
             i0 = torch.from_numpy(np.exp(rng.uniform(math.log(noise[0]), math.log(noise[1]),
                                                      nz)).astype(np.float32)).to(dev)
             b = _noisy(plan, x_true, i0, mu_max, gen)
-        ins.append(fbp(plan, b))
-        outs.append(sirt(plan, b, sirt_iters))
+        if input_iters > 0:
+            xi = sirt(plan, b, input_iters)
+            ins.append(xi)
+            outs.append(sirt(plan, b, sirt_iters - input_iters, x0=xi))
+        else:
+            ins.append(fbp(plan, b))
+            outs.append(sirt(plan, b, sirt_iters))
         sinos.append(b)
         truths.append(x_true)
     return torch.cat(ins), torch.cat(outs), torch.cat(sinos), torch.cat(truths)
@@ -162,17 +169,20 @@ def _time_ms(fn, reps=3):
     return (time.perf_counter() - t0) * 1e3 / reps
 
 
-def compare_starts(plan, net, b, plain_iters, max_iters):
-    """Time for plain SIRT from zero, from FBP and from U-Net(FBP) to reach the residual
-    plain SIRT from zero has after ``plain_iters`` iterations, on the (B, V, N) sinograms
-    b.  Wall time = start-up (FBP, network) + iterations x measured time per iteration
-    for the whole batch."""
+def compare_starts(plan, net, b, plain_iters, max_iters, input_iters=0):
+    """Time for plain SIRT from zero, from FBP and from the network's output to reach the
+    residual plain SIRT from zero has after ``plain_iters`` iterations, on the (B, V, N)
+    sinograms b.  The network's input is FBP(b) (``input_iters`` = 0) or the plain SIRT
+    iterate after ``input_iters`` iterations, whose cost is then part of its time.  Wall
+    time = start-up (FBP, input iterations, network) + iterations x measured time per
+    iteration for the whole batch."""
     nz = b.shape[0]
     x_plain = sirt(plan, b, plain_iters)
     target = residual(plan, x_plain, b) * (1.0 + 1e-9)
     zero = torch.zeros((nz, plan.npix, plan.npix), device=b.device)
     x_fbp = fbp(plan, b)
-    x_net = net(x_fbp)
+    x_in = sirt(plan, b, input_iters) if input_iters > 0 else x_fbp
+    x_net = net(x_in)
     k_zero, x_z = iterations_to(plan, b, zero, target, max_iters)
     k_fbp, _ = iterations_to(plan, b, x_fbp, target, max_iters)
     k_net, x_w = iterations_to(plan, b, x_net, target, max_iters)
@@ -181,7 +191,8 @@ def compare_starts(plan, net, b, plain_iters, max_iters):
     work = torch.empty_like(b)
     t_iter = _time_ms(lambda: plan.sirt_iterate(xx, b, R, C, work, pc_beta=0.0), reps=10)
     t_fbp = _time_ms(lambda: fbp(plan, b))
-    t_net = _time_ms(lambda: net(x_fbp))
+    t_net = _time_ms(lambda: net(x_in))
+    t_in = t_fbp if input_iters == 0 else input_iters * t_iter
     kz, kf, kn = int(k_zero.max()), int(k_fbp.max()), int(k_net.max())
     x_plain_k = sirt(plan, b, kz)
     rel_to_plain = float(torch.linalg.vector_norm((x_w - x_plain_k).double())
@@ -193,7 +204,8 @@ def compare_starts(plan, net, b, plain_iters, max_iters):
         "ms_per_iter": t_iter, "ms_fbp": t_fbp, "ms_unet": t_net,
         "seconds_zero": kz * t_iter / 1e3,
         "seconds_fbp": (t_fbp + kf * t_iter) / 1e3,
-        "seconds_unet": (t_fbp + t_net + kn * t_iter) / 1e3,
+        "input_iters": input_iters,
+        "seconds_unet": (t_in + t_net + kn * t_iter) / 1e3,
         "rel_l2_unet_vs_plain": rel_to_plain,
         "residual_fbp": float(residual(plan, x_fbp, b).max()),
         "residual_unet": float(residual(plan, x_net, b).max()),
@@ -229,6 +241,11 @@ def main(argv=None):
     ap.add_argument("--ellipses", type=int, default=10)
     ap.add_argument("--texture", type=float, default=0.15)
     ap.add_argument("--chunk", type=int, default=64, help="slices per SIRT launch")
+    ap.add_argument("--input-iters", type=int, default=0,
+                    help="network input: 0 = FBP, k = plain SIRT after k iterations")
+    ap.add_argument("--plain-iters", type=int, default=0,
+                    help="target residual: plain SIRT after this many iterations "
+                         "(default: --sirt-iters)")
     ap.add_argument("--save", default="")
     args = ap.parse_args(argv)
     plan = dd_plan(args.npix, args.nang)
@@ -236,7 +253,7 @@ def main(argv=None):
     t0 = time.perf_counter()
     xin, xout, _, _ = make_pairs(plan, args.pairs, args.sirt_iters, seed=1, noise=noise,
                                  n_ellipses=args.ellipses, texture=args.texture,
-                                 chunk=args.chunk)
+                                 chunk=args.chunk, input_iters=args.input_iters)
     torch.cuda.synchronize()
     t_data = time.perf_counter() - t0
     net = UNet(seed=0)
@@ -249,7 +266,8 @@ def main(argv=None):
     _, _, b_test, _ = make_pairs(plan, args.test, 1, seed=2, noise=noise,
                                  n_ellipses=args.ellipses, texture=args.texture,
                                  chunk=args.chunk)
-    res = compare_starts(plan, net, b_test, args.sirt_iters, args.max_iters)
+    res = compare_starts(plan, net, b_test, args.plain_iters or args.sirt_iters,
+                         args.max_iters, input_iters=args.input_iters)
     res.update({"pairs": args.pairs, "train_steps": args.steps, "seconds_data": t_data,
                 "seconds_train": t_train, "loss_first": float(losses[:10].mean()),
                 "loss_last": float(losses[-10:].mean()), "parameters": net.num_parameters()})
