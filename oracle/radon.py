@@ -5,6 +5,8 @@ The wrappers reproduce what the reference's Cython modules do around the C call:
 * ``radon_distance_driven.pyx:14-62``: degrees -> radians when ``max(angles) > 2*pi``,
   ``np.fft.fftshift`` of the angles, a zeroed float32 output, ``radon_dd(...)``.
 * ``radon_ray_driven.pyx:15-61``: the same without the fftshift.
+* ``radon_pixel_driven.pyx:15-73``: degrees -> radians by ``2*pi/180`` (sic) when
+  ``max(angles) > 2*pi``, fftshift, zeroed output, ``radon_pd(..., method, ...)``.
 
 Unlike the reference wrappers they never modify the caller's angle array in place.
 """
@@ -47,14 +49,22 @@ def lib():
             f = getattr(L, name)
             f.argtypes = [_f64p, ctypes.c_int, _f64p, ctypes.c_int, ctypes.c_int, _f64p]
             f.restype = ctypes.c_int
+        f = L.oracle_radon_pd
+        f.argtypes = [_f32p, ctypes.c_int, _f32p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                      _f32p, ctypes.c_int]
+        f.restype = ctypes.c_int
+        f = L.oracle_radon_pd_f64
+        f.argtypes = [_f64p, ctypes.c_int, _f64p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                      _f64p]
+        f.restype = ctypes.c_int
         _lib = L
     return _lib
 
 
-def _prep_angles(angles, shift):
+def _prep_angles(angles, shift, factor=np.pi / 180.0):
     a = np.array(angles, dtype=np.float32, copy=True).reshape(-1)
     if a.size and np.max(a) > 2 * np.pi:
-        a *= np.float32(np.pi / 180.0)  # in-place float32 multiply, as the .pyx does
+        a *= np.float32(factor)  # in-place float32 multiply, as the .pyx does
     if shift:
         a = np.fft.fftshift(a)
     return np.ascontiguousarray(a, dtype=np.float32)
@@ -110,9 +120,43 @@ def rd_back(sino, angles, mode=FAST):
     return _call("rd", sino, angles, 1, mode, shift=False)
 
 
+def _call_pd(x, angles, oper, mode, method):
+    L = lib()
+    a32 = _prep_angles(angles, True, 2 * np.pi / 180.0)   # radon_pixel_driven.pyx:27-30
+    nang = a32.size
+    npix = x.shape[1]
+    dt = np.float64 if mode == F64 else np.float32
+    src = np.ascontiguousarray(x, dtype=dt)
+    if oper == 0:
+        img, out = src, np.zeros((nang, npix), dtype=dt)
+        args = (img, npix, None, nang, 0, int(method), out)
+    else:
+        out = np.zeros((npix, npix), dtype=dt)
+        args = (out, npix, None, nang, 1, int(method), src)
+    if mode == F64:
+        rc = L.oracle_radon_pd_f64(args[0], npix, a32.astype(np.float64), nang, oper,
+                                   int(method), args[6])
+    else:
+        rc = L.oracle_radon_pd(args[0], npix, a32, nang, oper, int(method), args[6], mode)
+    if rc != 0:
+        raise MemoryError("oracle_radon_pd failed")
+    return out
+
+
+def pd_forward(image, angles, mode=FAST, method=1):
+    """Pixel-driven forward projection, (N,N) -> (V,N); sinogram row v holds view v of
+    the fftshift-ed angle list.  The facade always passes method 1
+    (class_projectors_radon.py:62)."""
+    return _call_pd(image, angles, 0, mode, method)
+
+
+def pd_back(sino, angles, mode=FAST, method=1):
+    return _call_pd(sino, angles, 1, mode, method)
+
+
 def forward(oper, image, angles, mode=FAST):
-    return {"dd": dd_forward, "rd": rd_forward}[oper](image, angles, mode)
+    return {"dd": dd_forward, "rd": rd_forward, "pd": pd_forward}[oper](image, angles, mode)
 
 
 def back(oper, sino, angles, mode=FAST):
-    return {"dd": dd_back, "rd": rd_back}[oper](sino, angles, mode)
+    return {"dd": dd_back, "rd": rd_back, "pd": pd_back}[oper](sino, angles, mode)
