@@ -10,7 +10,9 @@
  *   image : npix x npix, row-major (img[row][col])
  *   sino  : nang x npix; for the distance-driven projector row nang-1-v holds view v of
  *           the fftshift-ed angle list (radon_dd.c:79, radon_distance_driven.pyx:27);
- *           for the ray-driven projector row v holds view v (radon_rd.c:90).
+ *           for the ray-driven and pixel-driven projectors row v holds view v
+ *           (radon_rd.c:90, radon_pd.c:61; the pixel-driven wrapper fftshifts the angles,
+ *           radon_pixel_driven.pyx:30).
  * Batched calls take `nslices` images (stride npix*npix) and sinograms (stride nang*npix)
  * back to back.
  *
@@ -33,6 +35,7 @@ extern "C" {
 
 #define ITOMO_PROJ_DD 0       /* distance-driven, radon_dd.c */
 #define ITOMO_PROJ_RD 1       /* ray-driven, radon_rd.c */
+#define ITOMO_PROJ_PD 2       /* pixel-driven, radon_pd.c */
 
 #define ITOMO_ARITH_FAST 0    /* float arithmetic of the reference as built with its own
                                  flags (-O3 -ffast-math, create_module.py:14) */
@@ -59,6 +62,15 @@ int itomo_radon_dd(float *image, int npix, const float *angles, int nang, int op
                    float *sino, void *stream);
 int itomo_radon_rd(float *image, int npix, const float *angles, int nang, int oper,
                    float *sino, void *stream);
+/* Replaces `void radon_pd(float* image, int npix, float* angles, int nang, int oper,
+ * int method, float* sino)` (radon_pd.c:20), bound at radon_pixel_driven.pyx:7.
+ * method 0 = nearest bin, 1 = linear split between two bins (the facade always passes 1,
+ * class_projectors_radon.py:62,75).  Method 0 follows the reference's unchecked
+ * indexing, sino[v*npix + l]: a sample with l outside [0, npix) lands in the
+ * neighbouring sinogram row; flat indices outside the array (where the reference writes
+ * out of bounds) are skipped. */
+int itomo_radon_pd(float *image, int npix, const float *angles, int nang, int oper,
+                   int method, float *sino, void *stream);
 
 /* ---------------------------------------------------------------------------------
  * Plans: per-geometry constants (per-view sin/cos in the reference's float
@@ -71,6 +83,10 @@ int itomo_radon_rd(float *image, int npix, const float *angles, int nang, int op
  * ------------------------------------------------------------------------------- */
 int itomo_plan_create(itomo_plan **plan, int projector, int npix, const float *angles,
                       int nang, int arith);
+/* Pixel-driven plan with an explicit method (radon_pd.c:20 `method`); itomo_plan_create
+ * with ITOMO_PROJ_PD uses method 1, the facade's (class_projectors_radon.py:62). */
+int itomo_plan_create_pd(itomo_plan **plan, int npix, const float *angles, int nang,
+                         int method, int arith);
 int itomo_plan_destroy(itomo_plan *plan);
 /* Host copy of the per-view sin/cos the plan uses (for checking). */
 int itomo_plan_trig(const itomo_plan *plan, float *s_host, float *c_host);
@@ -81,7 +97,7 @@ int itomo_plan_trig(const itomo_plan *plan, float *s_host, float *c_host);
  * [0, ITOMO_BACK_VARIANTS): 64x64 tile with 4x4 pixels per thread / 32x32 with 2x2 /
  * 32x32 with 4x4.  Every shape produces bit-identical projections and iterates; FP64
  * monitor sums follow the block partition and may differ in the last bits.
- * Ray-driven plans accept only (0, 0).  Not to be called while work using the plan is
+ * Ray- and pixel-driven plans accept only (0, 0).  Not to be called while work using the plan is
  * queued on another stream. */
 #define ITOMO_FWD_VARIANTS 3
 #define ITOMO_BACK_VARIANTS 3

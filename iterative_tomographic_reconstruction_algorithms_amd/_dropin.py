@@ -1,7 +1,7 @@
 """Shared plumbing of the drop-in projector modules.
 
-The reference's Cython wrappers (radon_distance_driven.pyx, radon_ray_driven.pyx)
-take float32 C-contiguous numpy arrays; a wrong dtype, rank or layout raises
+The reference's Cython wrappers (radon_distance_driven.pyx, radon_ray_driven.pyx,
+radon_pixel_driven.pyx) take float32 C-contiguous numpy arrays; a wrong dtype, rank or layout raises
 ``ValueError`` from the buffer check and ``None`` raises ``TypeError``.  The drop-ins
 keep that contract for numpy input (the result comes back as numpy) and additionally
 accept float32 GPU tensors, which stay on the GPU.
@@ -17,13 +17,13 @@ _PLANS = collections.OrderedDict()
 _MAX_PLANS = 16
 
 
-def plan_for(oper, npix, c_angles, arith="fast"):
-    """Cached plan for (projector, npix, angles as the C receives them)."""
+def plan_for(oper, npix, c_angles, arith="fast", method=1):
+    """Cached plan for (projector, npix, angles as the C receives them[, method])."""
     a = np.ascontiguousarray(c_angles, dtype=np.float32)
-    key = (oper, int(npix), arith, a.tobytes())
+    key = (oper, int(npix), arith, int(method), a.tobytes())
     p = _PLANS.get(key)
     if p is None:
-        p = Plan(oper, npix, a, arith)
+        p = Plan(oper, npix, a, arith, method)
         _PLANS[key] = p
         if len(_PLANS) > _MAX_PLANS:
             _PLANS.popitem(last=False)[1].close()

@@ -34,21 +34,25 @@ class projectors:
         st["_plan"] = None
         return st
 
-    # C-order angles: dd (and pd/ss) fftshift (radon_distance_driven.pyx:27), rd not.
+    # C-order angles: dd and pd fftshift (radon_distance_driven.pyx:27,
+    # radon_pixel_driven.pyx:30), rd not.  The wrappers' degree test multiplies by pi/180
+    # (dd, rd) or 2*pi/180 (pd, radon_pixel_driven.pyx:27-28).
     def _c_angles(self):
         a = self.angles.copy()
-        if a.size and np.max(a) > 2 * np.pi:                # radon_*.pyx:25-26
-            a *= np.float32(np.pi / 180.0)
-        return np.fft.fftshift(a) if self.oper == "dd" else a
+        if a.size and np.max(a) > 2 * np.pi:
+            a *= np.float32((2 * np.pi if self.oper == "pd" else np.pi) / 180.0)
+        return np.fft.fftshift(a) if self.oper in ("dd", "pd") else a
 
     def plan(self, npix=None):
-        if self.oper not in ("dd", "rd"):
+        if self.oper not in ("dd", "rd", "pd"):
             raise NotImplementedError(
-                "projector %r has no GPU implementation yet (available: 'dd', 'rd')"
+                "projector %r has no GPU implementation yet (available: 'dd', 'rd', 'pd')"
                 % self.oper)
         npix = self.npix if npix is None else int(npix)
         if self._plan is None or self._plan.npix != npix:
-            self._plan = Plan(self.oper, npix, self._c_angles())
+            # the facade passes method 1 to the pixel-driven module
+            # (class_projectors_radon.py:62,75)
+            self._plan = Plan(self.oper, npix, self._c_angles(), method=1)
         return self._plan
 
     @staticmethod

@@ -43,13 +43,15 @@ class Plan:
     """Geometry of one projector on the GPU.
 
     ``angles`` are radians exactly as the reference C receives them: already
-    ``fftshift``-ed for the distance-driven projector (radon_distance_driven.pyx:27),
-    as given for the ray-driven one.  ``arith`` selects the float arithmetic: ``"fast"``
-    (the reference built with its own -ffast-math flags, the default) or ``"ieee"``
-    (the reference C as written).
+    ``fftshift``-ed for the distance- and pixel-driven projectors
+    (radon_distance_driven.pyx:27, radon_pixel_driven.pyx:30), as given for the
+    ray-driven one.  ``arith`` selects the float arithmetic: ``"fast"`` (the reference
+    built with its own -ffast-math flags, the default) or ``"ieee"`` (the reference C as
+    written).  ``method`` is the pixel-driven projector's bin rule (0 nearest, 1 linear,
+    radon_pd.c:57-85; the facade uses 1) and is ignored by the others.
     """
 
-    def __init__(self, oper, npix, angles, arith="fast"):
+    def __init__(self, oper, npix, angles, arith="fast", method=1):
         if oper not in L.PROJ:
             raise ValueError("projector %r not supported on the GPU path (have %s)"
                              % (oper, sorted(L.PROJ)))
@@ -58,12 +60,17 @@ class Plan:
         self.angles = np.ascontiguousarray(np.asarray(angles, dtype=np.float32).reshape(-1))
         self.nang = int(self.angles.size)
         self.arith = arith
+        self.method = int(method) if oper == "pd" else None
         device()
         h = ctypes.c_void_p()
-        L.check(L.lib().itomo_plan_create(
-            ctypes.byref(h), L.PROJ[oper], self.npix,
-            self.angles.ctypes.data_as(ctypes.c_void_p), self.nang, L.ARITH[arith]),
-            "plan_create")
+        aptr = self.angles.ctypes.data_as(ctypes.c_void_p)
+        if oper == "pd":
+            rc = L.lib().itomo_plan_create_pd(ctypes.byref(h), self.npix, aptr, self.nang,
+                                              self.method, L.ARITH[arith])
+        else:
+            rc = L.lib().itomo_plan_create(ctypes.byref(h), L.PROJ[oper], self.npix, aptr,
+                                           self.nang, L.ARITH[arith])
+        L.check(rc, "plan_create")
         self._h = h
 
     # -- lifetime -------------------------------------------------------------------

@@ -197,11 +197,18 @@ int elementwise_back_epilogue(int epi, const float* acc_img, float* x, const Epi
 int elementwise_fwd_epilogue(int epi, float* sino_io, const EpiArgs& e, int64_t nsino,
                              int nslices, hipStream_t st, int* blocks_per_slice);
 
-// projector launchers with epilogues (dd.hip / rd.hip)
+// projector launchers with epilogues (dd.hip)
 int dd_forward_epi(const Plan& p, int epi, int ones_input, const float* image, float* out,
                    int nslices, int accumulate, const EpiArgs& e, hipStream_t st,
                    int* blocks_per_slice);
 int dd_back_epi(const Plan& p, int epi, int ones_input, const float* sino, float* out,
+                int nslices, int accumulate, const EpiArgs& e, hipStream_t st,
+                int* blocks_per_slice);
+// pd.hip
+int pd_forward_epi(const Plan& p, int epi, int ones_input, const float* image, float* out,
+                   int nslices, int accumulate, const EpiArgs& e, hipStream_t st,
+                   int* blocks_per_slice);
+int pd_back_epi(const Plan& p, int epi, int ones_input, const float* sino, float* out,
                 int nslices, int accumulate, const EpiArgs& e, hipStream_t st,
                 int* blocks_per_slice);
 

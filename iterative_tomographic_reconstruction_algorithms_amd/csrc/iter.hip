@@ -177,16 +177,20 @@ int projector_forward_epi_any(const Plan& p, int epi, int ones, const float* ima
 {
     if (p.projector == ITOMO_PROJ_DD)
         return dd_forward_epi(p, epi, ones, image, out, nslices, 0, e, st, bps);
+    if (p.projector == ITOMO_PROJ_PD)
+        return pd_forward_epi(p, epi, ones, image, out, nslices, 0, e, st, bps);
     return rd_forward_epi(p, epi, ones, image, out, nslices, 0, e, st, bps);
 }
 
 // Back projection followed by a back-side epilogue writing `x`.  The ray-driven path
-// scatters into plan scratch first.
+// scatters into plan scratch first; the others fuse the epilogue into the gather.
 int projector_back_epi_any(const Plan& p, int epi, int ones, const float* sino, float* x,
                            int nslices, const EpiArgs& e, hipStream_t st, int* bps)
 {
     if (p.projector == ITOMO_PROJ_DD)
         return dd_back_epi(p, epi, ones, sino, x, nslices, 0, e, st, bps);
+    if (p.projector == ITOMO_PROJ_PD)
+        return pd_back_epi(p, epi, ones, sino, x, nslices, 0, e, st, bps);
     Scratch s;
     int rc = get_scratch(p, nslices, &s, st);
     if (rc) return rc;

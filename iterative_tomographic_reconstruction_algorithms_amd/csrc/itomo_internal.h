@@ -50,6 +50,23 @@ struct RdView {
     float pad;
 };
 
+// Per-view constants of the pixel-driven projector (radon_pd.c:29-55).
+enum PdFlags : int {
+    kPdLt = 1,    // (double)theta < pi/2
+    kPdGt = 2,    // (double)theta > pi/2
+    kPdZero = 4,  // theta == 0
+    kPdHalf = 8,  // (double)theta == pi/2 (never true for a float angle; kept as written)
+};
+
+struct PdView {
+    float s, c;   // sin, cos of the view as the reference computes them
+    float rc;     // 1/c (FAST form of x*s/c)
+    int flags;    // PdFlags
+    // Candidate ranges only (never in the arithmetic of a result): the sample distance
+    // is t ~ cabs*y - sdir*x; rsdir = 1/sdir, or 0 when sdir == 0.
+    double cabs, sdir, rsdir;
+};
+
 struct Plan {
     int projector;   // ITOMO_PROJ_*
     int npix;
@@ -60,6 +77,7 @@ struct Plan {
     float* d_lstar;  // distance-driven: [view][line] start of the dropped segment
     int fwd_var;     // distance-driven launch shapes (kFwdBins / kBackTiles index)
     int back_var;
+    int method;      // pixel-driven: 0 nearest bin, 1 linear split (radon_pd.c:57-85)
     // Device scratch owned by the plan (FP64 partial sums, ray-driven back-projection
     // image).  Grown on first use; a plan must not be used from two streams at once.
     mutable void* scratch;
@@ -102,16 +120,24 @@ inline size_t dd_back_blocks(int npix, int tile)
 
 inline size_t fwd_blocks_per_slice(const Plan& p)
 {
-    return p.projector == ITOMO_PROJ_DD
-               ? dd_fwd_slots(p.npix, p.nang, kFwdBins[p.fwd_var])
-               : (size_t)((p.npix + kRdBlock - 1) / kRdBlock) * p.nang;
+    if (p.projector == ITOMO_PROJ_DD) return dd_fwd_slots(p.npix, p.nang, kFwdBins[p.fwd_var]);
+    if (p.projector == ITOMO_PROJ_PD)  // partials come from the elementwise epilogue pass
+        return ((size_t)p.nang * p.npix + kEpiChunk - 1) / kEpiChunk;
+    return (size_t)((p.npix + kRdBlock - 1) / kRdBlock) * p.nang;
+}
+
+// Pixel-driven back projection: 64 x 4 pixels per block.
+inline size_t pd_back_blocks(int npix)
+{
+    return (size_t)((npix + 63) / 64) * (size_t)((npix + 3) / 4);
 }
 
 inline size_t back_blocks_per_slice(const Plan& p)
 {
     const size_t n2 = (size_t)p.npix * p.npix;
-    return p.projector == ITOMO_PROJ_DD ? dd_back_blocks(p.npix, kBackTiles[p.back_var][0])
-                                        : (n2 + kEpiChunk - 1) / kEpiChunk;
+    if (p.projector == ITOMO_PROJ_DD) return dd_back_blocks(p.npix, kBackTiles[p.back_var][0]);
+    if (p.projector == ITOMO_PROJ_PD) return pd_back_blocks(p.npix);
+    return (n2 + kEpiChunk - 1) / kEpiChunk;
 }
 
 // Per-call device scratch carved from the plan's buffer.  The layout depends only on
@@ -138,6 +164,10 @@ int dd_back(const Plan& p, const float* sino, float* image, int nslices,
 int rd_forward(const Plan& p, const float* image, float* sino, int nslices,
                int64_t img_stride, int64_t sino_stride, int accumulate, hipStream_t st);
 int rd_back(const Plan& p, const float* sino, float* image, int nslices,
+            int64_t sino_stride, int64_t img_stride, int accumulate, hipStream_t st);
+int pd_forward(const Plan& p, const float* image, float* sino, int nslices,
+               int64_t img_stride, int64_t sino_stride, int accumulate, hipStream_t st);
+int pd_back(const Plan& p, const float* sino, float* image, int nslices,
             int64_t sino_stride, int64_t img_stride, int accumulate, hipStream_t st);
 
 }  // namespace itomo

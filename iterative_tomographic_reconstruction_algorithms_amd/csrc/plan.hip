@@ -112,6 +112,21 @@ static int make_rd_view(float theta, int arith, RdView* g)
     return ITOMO_OK;
 }
 
+static void make_pd_view(float theta, int arith, PdView* g)
+{
+    float s, c;
+    view_trig(theta, arith, &s, &c);
+    std::memset(g, 0, sizeof *g);
+    g->s = s;
+    g->c = c;
+    g->rc = 1.0f / c;
+    g->flags = ((double)theta < kPi / 2 ? kPdLt : 0) | ((double)theta > kPi / 2 ? kPdGt : 0) |
+               (theta == 0.0f ? kPdZero : 0) | ((double)theta == kPi / 2 ? kPdHalf : 0);
+    g->cabs = std::fabs((double)c);
+    g->sdir = c < 0.0f ? -(double)s : (double)s;
+    g->rsdir = g->sdir != 0.0 ? 1.0 / g->sdir : 0.0;
+}
+
 int plan_scratch(const Plan& p, size_t bytes, void** ptr, hipStream_t st)
 {
     if (bytes > p.scratch_bytes) {
@@ -165,13 +180,16 @@ extern "C" {
 const char* itomo_version(void) { return "itomo 0.1.0 (gfx950)"; }
 const char* itomo_last_error(void) { return g_err; }
 
-int itomo_plan_create(itomo_plan** out, int projector, int npix, const float* angles,
-                      int nang, int arith)
+static int plan_create(itomo_plan** out, int projector, int npix, const float* angles,
+                       int nang, int method, int arith)
 {
     if (!out) return set_error(ITOMO_EINVAL, "plan_create: null output pointer");
     *out = nullptr;
-    if (projector != ITOMO_PROJ_DD && projector != ITOMO_PROJ_RD)
+    if (projector != ITOMO_PROJ_DD && projector != ITOMO_PROJ_RD && projector != ITOMO_PROJ_PD)
         return set_error(ITOMO_EINVAL, "plan_create: unknown projector %d", projector);
+    if (method != 0 && method != 1)
+        return set_error(ITOMO_EINVAL, "plan_create: method %d is neither 0 (nearest) nor 1 "
+                                       "(linear)", method);
     if (arith != ITOMO_ARITH_FAST && arith != ITOMO_ARITH_IEEE)
         return set_error(ITOMO_EINVAL, "plan_create: unknown arithmetic %d", arith);
     // short indices in radon_dd.c:19-20 bound npix to 32766; keep well inside int math
@@ -189,6 +207,7 @@ int itomo_plan_create(itomo_plan** out, int projector, int npix, const float* an
     p->npix = npix;
     p->nang = nang;
     p->arith = arith;
+    p->method = method;
     size_t bytes = 0;
     int st = ITOMO_OK;
     if (projector == ITOMO_PROJ_DD) {
@@ -197,6 +216,12 @@ int itomo_plan_create(itomo_plan** out, int projector, int npix, const float* an
         if (!h) { delete p; return set_error(ITOMO_ENOMEM, "plan_create: out of host memory"); }
         for (int v = 0; v < nang && st == ITOMO_OK; ++v)
             st = make_dd_view(angles[v], arith, nang - 1 - v, &h[v]);
+        p->h_geom = h;
+    } else if (projector == ITOMO_PROJ_PD) {
+        bytes = sizeof(PdView) * (size_t)nang;
+        PdView* h = (PdView*)std::malloc(bytes);
+        if (!h) { delete p; return set_error(ITOMO_ENOMEM, "plan_create: out of host memory"); }
+        for (int v = 0; v < nang; ++v) make_pd_view(angles[v], arith, &h[v]);
         p->h_geom = h;
     } else {
         bytes = sizeof(RdView) * (size_t)nang;
@@ -227,6 +252,18 @@ int itomo_plan_create(itomo_plan** out, int projector, int npix, const float* an
     return ITOMO_OK;
 }
 
+int itomo_plan_create(itomo_plan** out, int projector, int npix, const float* angles,
+                      int nang, int arith)
+{
+    return plan_create(out, projector, npix, angles, nang, 1, arith);
+}
+
+int itomo_plan_create_pd(itomo_plan** out, int npix, const float* angles, int nang,
+                         int method, int arith)
+{
+    return plan_create(out, ITOMO_PROJ_PD, npix, angles, nang, method, arith);
+}
+
 int itomo_plan_destroy(itomo_plan* p)
 {
     if (!p) return ITOMO_OK;
@@ -250,6 +287,9 @@ int itomo_plan_trig(const itomo_plan* p, float* s_host, float* c_host)
     if (!p || !s_host || !c_host) return set_error(ITOMO_EINVAL, "plan_trig: null pointer");
     if (p->projector == ITOMO_PROJ_RD) {
         const RdView* g = (const RdView*)p->h_geom;
+        for (int v = 0; v < p->nang; ++v) { s_host[v] = g[v].s; c_host[v] = g[v].c; }
+    } else if (p->projector == ITOMO_PROJ_PD) {
+        const PdView* g = (const PdView*)p->h_geom;
         for (int v = 0; v < p->nang; ++v) { s_host[v] = g[v].s; c_host[v] = g[v].c; }
     } else {
         const DdView* g = (const DdView*)p->h_geom;
@@ -299,6 +339,8 @@ int itomo_forward(const itomo_plan* p, const float* image, float* sino, int nsli
     hipStream_t s = (hipStream_t)stream;
     if (p->projector == ITOMO_PROJ_DD)
         return dd_forward(*p, image, sino, nslices, is, ss, accumulate, s);
+    if (p->projector == ITOMO_PROJ_PD)
+        return pd_forward(*p, image, sino, nslices, is, ss, accumulate, s);
     return rd_forward(*p, image, sino, nslices, is, ss, accumulate, s);
 }
 
@@ -311,16 +353,18 @@ int itomo_back(const itomo_plan* p, const float* sino, float* image, int nslices
     hipStream_t s = (hipStream_t)stream;
     if (p->projector == ITOMO_PROJ_DD)
         return dd_back(*p, sino, image, nslices, ss, is, accumulate, s);
+    if (p->projector == ITOMO_PROJ_PD)
+        return pd_back(*p, sino, image, nslices, ss, is, accumulate, s);
     return rd_back(*p, sino, image, nslices, ss, is, accumulate, s);
 }
 
 static int one_shot(int projector, float* image, int npix, const float* angles, int nang,
-                    int oper, float* sino, void* stream)
+                    int oper, int method, float* sino, void* stream)
 {
     if (oper != 0 && oper != 1)
         return set_error(ITOMO_EINVAL, "radon: oper must be 0 (forward) or 1 (back)");
     itomo_plan* p = nullptr;
-    int st = itomo_plan_create(&p, projector, npix, angles, nang, ITOMO_ARITH_FAST);
+    int st = plan_create(&p, projector, npix, angles, nang, method, ITOMO_ARITH_FAST);
     if (st) return st;
     st = oper == 0 ? itomo_forward(p, image, sino, 1, 1, stream)
                    : itomo_back(p, sino, image, 1, 1, stream);
@@ -333,13 +377,19 @@ static int one_shot(int projector, float* image, int npix, const float* angles, 
 int itomo_radon_dd(float* image, int npix, const float* angles, int nang, int oper,
                    float* sino, void* stream)
 {
-    return one_shot(ITOMO_PROJ_DD, image, npix, angles, nang, oper, sino, stream);
+    return one_shot(ITOMO_PROJ_DD, image, npix, angles, nang, oper, 1, sino, stream);
 }
 
 int itomo_radon_rd(float* image, int npix, const float* angles, int nang, int oper,
                    float* sino, void* stream)
 {
-    return one_shot(ITOMO_PROJ_RD, image, npix, angles, nang, oper, sino, stream);
+    return one_shot(ITOMO_PROJ_RD, image, npix, angles, nang, oper, 1, sino, stream);
+}
+
+int itomo_radon_pd(float* image, int npix, const float* angles, int nang, int oper, int method,
+                   float* sino, void* stream)
+{
+    return one_shot(ITOMO_PROJ_PD, image, npix, angles, nang, oper, method, sino, stream);
 }
 
 }  // extern "C"

@@ -120,18 +120,20 @@ def rd_back(sino, angles, mode=FAST):
     return _call("rd", sino, angles, 1, mode, shift=False)
 
 
-def _call_pd(x, angles, oper, mode, method):
+def _call_pd(x, angles, oper, mode, method, init=None):
     L = lib()
     a32 = _prep_angles(angles, True, 2 * np.pi / 180.0)   # radon_pixel_driven.pyx:27-30
     nang = a32.size
     npix = x.shape[1]
     dt = np.float64 if mode == F64 else np.float32
     src = np.ascontiguousarray(x, dtype=dt)
+    shape = (nang, npix) if oper == 0 else (npix, npix)
+    # the C adds into its output; the wrapper hands it zeros, `init` another start
+    out = (np.zeros(shape, dtype=dt) if init is None
+           else np.array(init, dtype=dt, copy=True).reshape(shape))
     if oper == 0:
-        img, out = src, np.zeros((nang, npix), dtype=dt)
-        args = (img, npix, None, nang, 0, int(method), out)
+        args = (src, npix, None, nang, 0, int(method), out)
     else:
-        out = np.zeros((npix, npix), dtype=dt)
         args = (out, npix, None, nang, 1, int(method), src)
     if mode == F64:
         rc = L.oracle_radon_pd_f64(args[0], npix, a32.astype(np.float64), nang, oper,
@@ -143,15 +145,15 @@ def _call_pd(x, angles, oper, mode, method):
     return out
 
 
-def pd_forward(image, angles, mode=FAST, method=1):
+def pd_forward(image, angles, mode=FAST, method=1, init=None):
     """Pixel-driven forward projection, (N,N) -> (V,N); sinogram row v holds view v of
     the fftshift-ed angle list.  The facade always passes method 1
-    (class_projectors_radon.py:62)."""
-    return _call_pd(image, angles, 0, mode, method)
+    (class_projectors_radon.py:62).  ``init``: the sinogram the C adds into (zeros)."""
+    return _call_pd(image, angles, 0, mode, method, init)
 
 
-def pd_back(sino, angles, mode=FAST, method=1):
-    return _call_pd(sino, angles, 1, mode, method)
+def pd_back(sino, angles, mode=FAST, method=1, init=None):
+    return _call_pd(sino, angles, 1, mode, method, init)
 
 
 def forward(oper, image, angles, mode=FAST):

@@ -59,16 +59,25 @@ def _angles(angles, proj):
     return np.ascontiguousarray(a, dtype=np.float32)
 
 
+# The pixel-driven C with method 0 indexes sino[v*npix + l] without a range check
+# (radon_pd.c:58-63); l stays within (-npix/4, 5*npix/4), so one zero guard row on each
+# side of the sinogram keeps every access inside the buffer handed to the C.  Forward
+# contributions that land in the guards are dropped; in the back projection the guards
+# read as zeros, i.e. they contribute nothing.
+_GUARD = 1
+
+
 def forward(proj, image, angles, flags="fast", method=1):
     f = _fn(proj, flags)
     img = np.ascontiguousarray(image, dtype=np.float32)
     npix = img.shape[0]
     a = _angles(angles, proj)
-    sino = np.zeros((a.size, npix), dtype=np.float32)
     if proj in ("pd", "ss"):
-        f(img, npix, a, a.size, 0, method, sino)
-    else:
-        f(img, npix, a, a.size, 0, sino)
+        g = np.zeros((a.size + 2 * _GUARD, npix), dtype=np.float32)
+        f(img, npix, a, a.size, 0, method, g[_GUARD:])
+        return g[_GUARD:_GUARD + a.size].copy()
+    sino = np.zeros((a.size, npix), dtype=np.float32)
+    f(img, npix, a, a.size, 0, sino)
     return sino
 
 
@@ -79,7 +88,9 @@ def back(proj, sino, angles, flags="fast", method=1):
     a = _angles(angles, proj)
     image = np.zeros((npix, npix), dtype=np.float32)
     if proj in ("pd", "ss"):
-        f(image, npix, a, nang, 1, method, s)
+        g = np.zeros((nang + 2 * _GUARD, npix), dtype=np.float32)
+        g[_GUARD:_GUARD + nang] = s
+        f(image, npix, a, nang, 1, method, g[_GUARD:])
     else:
         f(image, npix, a, nang, 1, s)
     return image

@@ -1,7 +1,7 @@
 """Write the golden projector vectors in tests/golden/radon_ref.npz.
 
 Inputs are seeded phantoms / sinograms; expected outputs come from the reference's own
-radon C (radon_dd.c, radon_rd.c) compiled from the reference sources by
+radon C (radon_dd.c, radon_rd.c; radon_pd.c into radon_pd_ref.npz) compiled from the reference sources by
 ``make -C oracle ref`` into ``oracle/_ref`` -- both with the reference's flags
 (``fast``: -O3 -ffast-math) and with IEEE flags (``strict``).  The angle arrays stored
 are what the reference facade passes to the Cython wrappers (float32 radians,
@@ -23,6 +23,7 @@ from iterative_tomographic_reconstruction_algorithms_amd.phantom import (  # noq
     angles_deg, ellipses)
 
 OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "radon_ref.npz")
+OUT_PD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "radon_pd_ref.npz")
 
 
 def facade_angles(deg):
@@ -78,6 +79,48 @@ def main():
         data[key + "/back_" + flags] = ref.back("dd", sino, ang, flags)
     np.savez_compressed(OUT, **data)
     print("wrote %s (%d arrays, %.1f MB)" % (OUT, len(data), os.path.getsize(OUT) / 1e6))
+    pixel_driven()
+
+
+def pd_cases():
+    out = []
+    for n, v in ((8, 1), (8, 4), (16, 36), (64, 36), (63, 37), (64, 305), (128, 180)):
+        out.append(("grid", n, facade_angles(angles_deg(v))))
+    for name, theta in (("t0", 0.0), ("t45", np.pi / 4), ("t90m", np.pi / 2 - 1e-3),
+                        ("t90", np.pi / 2), ("t90p", np.pi / 2 + 1e-3),
+                        ("t135", 3 * np.pi / 4), ("t180", np.pi), ("t234", 1.3 * np.pi),
+                        ("tneg", -0.4)):
+        out.append((name, 64, np.array([theta], dtype=np.float32)))
+    # degrees: the wrapper multiplies by 2*pi/180 (radon_pixel_driven.pyx:27-28)
+    out.append(("deg", 32, angles_deg(12).astype(np.float32)))
+    return out
+
+
+def pixel_driven():
+    """radon_pd.c with method 0 (nearest bin) and 1 (linear, the facade's), through the
+    wrapper's angle handling (oracle.ref applies the 2*pi/180 degree factor and the
+    fftshift of radon_pixel_driven.pyx:27-30).  Half of the images are nonzero up to the
+    corners, so method 0's samples that overflow into the neighbouring sinogram row are
+    covered."""
+    if not ref.available("pd"):
+        sys.exit("oracle/_ref is not built: run `make -C oracle ref` first")
+    data = {}
+    for k, (name, n, ang) in enumerate(pd_cases()):
+        rng = np.random.default_rng(400 + k)
+        img = (ellipses(n, 10, seed=500 + k) if k % 2 == 0
+               else (rng.random((n, n)) - 0.25).astype(np.float32))
+        sino = rng.random((ang.size, n)).astype(np.float32)
+        for method in (0, 1):
+            key = "pd_%s_%d_%d_m%d" % (name, n, ang.size, method)
+            data[key + "/angles"] = ang
+            data[key + "/image"] = img
+            data[key + "/sino"] = sino
+            for flags in ("fast", "strict"):
+                data[key + "/fwd_" + flags] = ref.forward("pd", img, ang, flags, method)
+                data[key + "/back_" + flags] = ref.back("pd", sino, ang, flags, method)
+    np.savez_compressed(OUT_PD, **data)
+    print("wrote %s (%d arrays, %.1f MB)" % (OUT_PD, len(data),
+                                             os.path.getsize(OUT_PD) / 1e6))
 
 
 if __name__ == "__main__":

@@ -59,3 +59,8 @@ def test_plan_rejects_bad_arguments_without_touching_the_gpu():
     bad = (ctypes.c_float * 1)(4.0)
     rc = L.itomo_plan_create(ctypes.byref(h), 0, 16, ctypes.cast(bad, ctypes.c_void_p), 1, 0)
     assert rc == -2 and b"sin <= 0" in L.itomo_last_error()
+    # pixel-driven: the bin rule must be 0 (nearest) or 1 (linear)
+    rc = L.itomo_plan_create_pd(ctypes.byref(h), 16, ctypes.cast(ang, ctypes.c_void_p), 2, 2, 0)
+    assert rc == -1 and b"method" in L.itomo_last_error()
+    rc = L.itomo_radon_pd(None, 16, ctypes.cast(ang, ctypes.c_void_p), 2, 2, 0, None, None)
+    assert rc == -1
