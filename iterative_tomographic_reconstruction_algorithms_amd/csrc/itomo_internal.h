@@ -63,8 +63,9 @@ struct PdView {
     float rc;     // 1/c (FAST form of x*s/c)
     int flags;    // PdFlags
     // Candidate ranges only (never in the arithmetic of a result): the sample distance
-    // is t ~ cabs*y - sdir*x; rsdir = 1/sdir, or 0 when sdir == 0.
-    double cabs, sdir, rsdir;
+    // is t ~ cabs*y - sdir*x; rsdir = 1/sdir, or 0 when sdir == 0; the two sample
+    // points of a pixel sit at +-dshift = +-|cabs - sdir|/4 from its centre's t.
+    double cabs, sdir, rsdir, dshift;
 };
 
 struct Plan {

@@ -14,10 +14,10 @@
 // reference's order and the results are bit-identical to the reference's, with no
 // atomics:
 //   forward: one thread per (view, bin) walks the image rows i upwards and, in each row,
-//            the columns whose pixel centre lies within 1 + 1/4*sqrt(2) + margin of the
-//            bin's line (t(x0, y0) = |c| y0 - sign(c) s x0 to float rounding, a bound
-//            fixed in double per row), evaluating every candidate sample exactly as
-//            the reference does and keeping those that land in the bin;
+//            the columns whose pixel centre lies within 1 + |(|c| - sign(c) s)|/4 +
+//            margin of the bin's line (t(x0, y0) = |c| y0 - sign(c) s x0 to float
+//            rounding, a bound fixed in double per row), evaluating every candidate
+//            sample exactly as the reference does and keeping those that land in the bin;
 //   back:    one thread per pixel walks the views in order.
 // Method 0 has no range check in the reference (radon_pd.c:58-63): a sample whose bin
 // l lies outside [0, npix) lands at flat index v*npix + l, i.e. in a neighbouring
@@ -107,10 +107,13 @@ __device__ __forceinline__ float pd_fwd_pixel(const PdView& g, int nh, float nhf
             const PdSplit s = pd_split<ARITH>(nhf, t);
             const float ef = (float)e;
             const bool hl = s.lf == ef, hu = s.uf == ef;
-#pragma unroll
-            for (int rep = 0; rep < 2; ++rep) {
-                if (hl) acc = pd_acc(acc, s.wl, px);
-                if (hu) acc = pd_acc(acc, s.wu, px);
+            // lf == uf only for an integer t1, where both weights are 0: one zero-weight
+            // term then stands for the reference's two (x + 0 = x; a non-finite pixel
+            // gives NaN either way)
+            const float w = hl ? s.wl : s.wu;
+            if (hl || hu) {
+                acc = pd_acc(acc, w, px);
+                acc = pd_acc(acc, w, px);
             }
         }
     }
@@ -124,35 +127,35 @@ __device__ float pd_fwd_bin(const PdView& g, int npix, const float* __restrict__
 {
     const int nh = npix >> 1;
     const float nhf = (float)nh;
-    // |t - t(x0, y0)| <= sqrt(2)/4 for both sample points; float rounding of t, of the
-    // sign test and of nh - t stays far below npix * 2^-18.
-    const double half = 1.0 + 0.3536 + 0.01 + npix * 0x1p-18;
+    // The samples of a pixel sit at t(x0, y0) +- dshift; float rounding of t, of the sign
+    // test and of nh - t stays far below npix * 2^-18.
+    const double half = 1.0 + g.dshift + 0.01 + npix * 0x1p-18;
     const double mid = (double)(nh - e);
     const double tl = mid - half, th = mid + half;
     const double C = g.cabs, S = g.sdir;
     const double xa = 0.5 - nh, xb = (double)(npix - 1 - nh) + 0.5;
     const double smin = fmin(xa * S, xb * S), smax = fmax(xa * S, xb * S);
-    // rows whose centre line can meet the band: C*y0 in (tl + smin, th + smax)
+    // rows whose centre line can meet the band: C*y0 in (tl + smin, th + smax); the
+    // margin is inside the band, so the integer rounding can be tight
     int i0 = 0, i1 = npix - 1;
     if (C > 0.0) {
         const double ylo = (tl + smin) / C + (nh - 0.5), yhi = (th + smax) / C + (nh - 0.5);
-        i0 = (int)fmin(fmax(0.0, floor(ylo)), (double)npix);
-        i1 = (int)fmax(-1.0, fmin((double)(npix - 1), ceil(yhi)));
+        i0 = (int)fmin(fmax(0.0, ceil(ylo)), (double)npix);
+        i1 = (int)fmax(-1.0, fmin((double)(npix - 1), floor(yhi)));
     }
+    // column range of row i: x0 between (C*y0 - th)/S and (C*y0 - tl)/S, in column units
+    // j = x0 + nh - 1/2: j_lo/hi = a*y0 + b_lo/hi
+    const double rs = g.rsdir, a = C * rs, off = nh - 0.5;
+    const double b_lo = (rs > 0.0 ? -th : -tl) * rs + off;
+    const double b_hi = (rs > 0.0 ? -tl : -th) * rs + off;
     for (int i = i0; i <= i1; ++i) {
         const float y0 = __fadd_rn((float)(i - nh), 0.5f);
-        const double yc = (double)y0 * C;
         int j0, j1;
-        if (g.rsdir != 0.0) {
-            double xl = (yc - th) * g.rsdir + (nh - 0.5), xh = (yc - tl) * g.rsdir + (nh - 0.5);
-            if (g.rsdir < 0.0) {
-                const double tmp = xl;
-                xl = xh;
-                xh = tmp;
-            }
-            j0 = (int)fmin(fmax(0.0, floor(xl)), (double)npix);
-            j1 = (int)fmax(-1.0, fmin((double)(npix - 1), ceil(xh)));
+        if (rs != 0.0) {
+            j0 = (int)fmin(fmax(0.0, ceil(fma((double)y0, a, b_lo))), (double)npix);
+            j1 = (int)fmax(-1.0, fmin((double)(npix - 1), floor(fma((double)y0, a, b_hi))));
         } else {
+            const double yc = (double)y0 * C;
             const bool in = yc > tl && yc < th;
             j0 = in ? 0 : 1;
             j1 = in ? npix - 1 : 0;

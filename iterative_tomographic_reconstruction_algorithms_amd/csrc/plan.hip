@@ -125,6 +125,7 @@ static void make_pd_view(float theta, int arith, PdView* g)
     g->cabs = std::fabs((double)c);
     g->sdir = c < 0.0f ? -(double)s : (double)s;
     g->rsdir = g->sdir != 0.0 ? 1.0 / g->sdir : 0.0;
+    g->dshift = 0.25 * std::fabs(g->cabs - g->sdir);
 }
 
 int plan_scratch(const Plan& p, size_t bytes, void** ptr, hipStream_t st)
