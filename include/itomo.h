@@ -12,7 +12,8 @@
  *           the fftshift-ed angle list (radon_dd.c:79, radon_distance_driven.pyx:27);
  *           for the ray-driven and pixel-driven projectors row v holds view v
  *           (radon_rd.c:90, radon_pd.c:61; the pixel-driven wrapper fftshifts the angles,
- *           radon_pixel_driven.pyx:30).
+ *           radon_pixel_driven.pyx:30); for the slant-stacking projector row nang-1-v
+ *           holds view v (radon_ss.c:111; its wrapper does not fftshift).
  * Batched calls take `nslices` images (stride npix*npix) and sinograms (stride nang*npix)
  * back to back.
  *
@@ -36,6 +37,7 @@ extern "C" {
 #define ITOMO_PROJ_DD 0       /* distance-driven, radon_dd.c */
 #define ITOMO_PROJ_RD 1       /* ray-driven, radon_rd.c */
 #define ITOMO_PROJ_PD 2       /* pixel-driven, radon_pd.c */
+#define ITOMO_PROJ_SS 3       /* slant-stacking, radon_ss.c */
 
 #define ITOMO_ARITH_FAST 0    /* float arithmetic of the reference as built with its own
                                  flags (-O3 -ffast-math, create_module.py:14) */
@@ -71,6 +73,15 @@ int itomo_radon_rd(float *image, int npix, const float *angles, int nang, int op
  * out of bounds) are skipped. */
 int itomo_radon_pd(float *image, int npix, const float *angles, int nang, int oper,
                    int method, float *sino, void *stream);
+/* Replaces `void radon_ss(float* image, int npix, float* angles, int nang, int oper,
+ * int method, float* sino)` (radon_ss.c:93), bound at radon_slant_stacking.pyx:7.
+ * method 0 = nearest pixel, 1 = linear between two pixels (the facade passes 1,
+ * class_projectors_radon.py:68,81).  For lines that miss the image square (|t| close to
+ * npix/2) the reference's ray_limits yields a step range outside the image and its loop
+ * indexes outside the image rows (out-of-bounds reads and writes); here samples outside
+ * the image are skipped, which equals the reference wherever its samples stay inside. */
+int itomo_radon_ss(float *image, int npix, const float *angles, int nang, int oper,
+                   int method, float *sino, void *stream);
 
 /* ---------------------------------------------------------------------------------
  * Plans: per-geometry constants (per-view sin/cos in the reference's float
@@ -87,6 +98,11 @@ int itomo_plan_create(itomo_plan **plan, int projector, int npix, const float *a
  * with ITOMO_PROJ_PD uses method 1, the facade's (class_projectors_radon.py:62). */
 int itomo_plan_create_pd(itomo_plan **plan, int npix, const float *angles, int nang,
                          int method, int arith);
+/* Slant-stacking plan with an explicit method (radon_ss.c:93 `method`); itomo_plan_create
+ * with ITOMO_PROJ_SS uses method 1, the facade's (class_projectors_radon.py:68).  The plan
+ * holds a per-(view, bin) step-range table (8 bytes per bin). */
+int itomo_plan_create_ss(itomo_plan **plan, int npix, const float *angles, int nang,
+                         int method, int arith);
 int itomo_plan_destroy(itomo_plan *plan);
 /* Host copy of the per-view sin/cos the plan uses (for checking). */
 int itomo_plan_trig(const itomo_plan *plan, float *s_host, float *c_host);
@@ -97,7 +113,7 @@ int itomo_plan_trig(const itomo_plan *plan, float *s_host, float *c_host);
  * [0, ITOMO_BACK_VARIANTS): 64x64 tile with 4x4 pixels per thread / 32x32 with 2x2 /
  * 32x32 with 4x4.  Every shape produces bit-identical projections and iterates; FP64
  * monitor sums follow the block partition and may differ in the last bits.
- * Ray- and pixel-driven plans accept only (0, 0).  Not to be called while work using the plan is
+ * Ray-, pixel-driven and slant-stacking plans accept only (0, 0).  Not to be called while work using the plan is
  * queued on another stream. */
 #define ITOMO_FWD_VARIANTS 3
 #define ITOMO_BACK_VARIANTS 3

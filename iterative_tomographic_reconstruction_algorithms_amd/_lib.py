@@ -10,7 +10,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libitomo.so")
 
 ITOMO_OK = 0
-PROJ = {"dd": 0, "rd": 1, "pd": 2}
+PROJ = {"dd": 0, "rd": 1, "pd": 2, "ss": 3}
 ARITH = {"fast": 0, "ieee": 1}
 DOT_WORK = 3072
 
@@ -32,8 +32,10 @@ _SIGS = {
     "itomo_radon_dd": ([_vp, _i, _vp, _i, _i, _vp, _vp], _i),
     "itomo_radon_rd": ([_vp, _i, _vp, _i, _i, _vp, _vp], _i),
     "itomo_radon_pd": ([_vp, _i, _vp, _i, _i, _i, _vp, _vp], _i),
+    "itomo_radon_ss": ([_vp, _i, _vp, _i, _i, _i, _vp, _vp], _i),
     "itomo_plan_create": ([ctypes.POINTER(_vp), _i, _i, _vp, _i, _i], _i),
     "itomo_plan_create_pd": ([ctypes.POINTER(_vp), _i, _vp, _i, _i, _i], _i),
+    "itomo_plan_create_ss": ([ctypes.POINTER(_vp), _i, _vp, _i, _i, _i], _i),
     "itomo_plan_destroy": ([_vp], _i),
     "itomo_plan_trig": ([_vp, _vp, _vp], _i),
     "itomo_plan_set_shapes": ([_vp, _i, _i], _i),

@@ -35,23 +35,23 @@ class projectors:
         return st
 
     # C-order angles: dd and pd fftshift (radon_distance_driven.pyx:27,
-    # radon_pixel_driven.pyx:30), rd not.  The wrappers' degree test multiplies by pi/180
-    # (dd, rd) or 2*pi/180 (pd, radon_pixel_driven.pyx:27-28).
+    # radon_pixel_driven.pyx:30), rd and ss not.  The wrappers' degree test multiplies by
+    # pi/180 (dd, rd) or 2*pi/180 (pd, ss: radon_pixel_driven.pyx:27-28,
+    # radon_slant_stacking.pyx:27-28).
     def _c_angles(self):
         a = self.angles.copy()
         if a.size and np.max(a) > 2 * np.pi:
-            a *= np.float32((2 * np.pi if self.oper == "pd" else np.pi) / 180.0)
+            a *= np.float32((2 * np.pi if self.oper in ("pd", "ss") else np.pi) / 180.0)
         return np.fft.fftshift(a) if self.oper in ("dd", "pd") else a
 
     def plan(self, npix=None):
-        if self.oper not in ("dd", "rd", "pd"):
-            raise NotImplementedError(
-                "projector %r has no GPU implementation yet (available: 'dd', 'rd', 'pd')"
-                % self.oper)
+        if self.oper not in ("dd", "rd", "pd", "ss"):
+            raise NotImplementedError("projector %r is not one of 'pd', 'rd', 'dd', 'ss'"
+                                      % self.oper)
         npix = self.npix if npix is None else int(npix)
         if self._plan is None or self._plan.npix != npix:
-            # the facade passes method 1 to the pixel-driven module
-            # (class_projectors_radon.py:62,75)
+            # the facade passes method 1 to the pixel-driven and slant-stacking modules
+            # (class_projectors_radon.py:62,68,75,81)
             self._plan = Plan(self.oper, npix, self._c_angles(), method=1)
         return self._plan
 

@@ -45,10 +45,11 @@ class Plan:
     ``angles`` are radians exactly as the reference C receives them: already
     ``fftshift``-ed for the distance- and pixel-driven projectors
     (radon_distance_driven.pyx:27, radon_pixel_driven.pyx:30), as given for the
-    ray-driven one.  ``arith`` selects the float arithmetic: ``"fast"`` (the reference
+    ray-driven and slant-stacking ones.  ``arith`` selects the float arithmetic: ``"fast"`` (the reference
     built with its own -ffast-math flags, the default) or ``"ieee"`` (the reference C as
-    written).  ``method`` is the pixel-driven projector's bin rule (0 nearest, 1 linear,
-    radon_pd.c:57-85; the facade uses 1) and is ignored by the others.
+    written).  ``method`` is the pixel-driven and slant-stacking projectors' interpolation
+    (0 nearest, 1 linear, radon_pd.c:57-85, radon_ss.c:123-190; the facade uses 1) and is
+    ignored by the others.
     """
 
     def __init__(self, oper, npix, angles, arith="fast", method=1):
@@ -60,13 +61,14 @@ class Plan:
         self.angles = np.ascontiguousarray(np.asarray(angles, dtype=np.float32).reshape(-1))
         self.nang = int(self.angles.size)
         self.arith = arith
-        self.method = int(method) if oper == "pd" else None
+        self.method = int(method) if oper in ("pd", "ss") else None
         device()
         h = ctypes.c_void_p()
         aptr = self.angles.ctypes.data_as(ctypes.c_void_p)
-        if oper == "pd":
-            rc = L.lib().itomo_plan_create_pd(ctypes.byref(h), self.npix, aptr, self.nang,
-                                              self.method, L.ARITH[arith])
+        if oper in ("pd", "ss"):
+            create = getattr(L.lib(), "itomo_plan_create_" + oper)
+            rc = create(ctypes.byref(h), self.npix, aptr, self.nang, self.method,
+                        L.ARITH[arith])
         else:
             rc = L.lib().itomo_plan_create(ctypes.byref(h), L.PROJ[oper], self.npix, aptr,
                                            self.nang, L.ARITH[arith])

@@ -211,5 +211,12 @@ int pd_forward_epi(const Plan& p, int epi, int ones_input, const float* image, f
 int pd_back_epi(const Plan& p, int epi, int ones_input, const float* sino, float* out,
                 int nslices, int accumulate, const EpiArgs& e, hipStream_t st,
                 int* blocks_per_slice);
+// ss.hip
+int ss_forward_epi(const Plan& p, int epi, int ones_input, const float* image, float* out,
+                   int nslices, int accumulate, const EpiArgs& e, hipStream_t st,
+                   int* blocks_per_slice);
+int ss_back_epi(const Plan& p, int epi, int ones_input, const float* sino, float* out,
+                int nslices, int accumulate, const EpiArgs& e, hipStream_t st,
+                int* blocks_per_slice);
 
 }  // namespace itomo

@@ -179,6 +179,8 @@ int projector_forward_epi_any(const Plan& p, int epi, int ones, const float* ima
         return dd_forward_epi(p, epi, ones, image, out, nslices, 0, e, st, bps);
     if (p.projector == ITOMO_PROJ_PD)
         return pd_forward_epi(p, epi, ones, image, out, nslices, 0, e, st, bps);
+    if (p.projector == ITOMO_PROJ_SS)
+        return ss_forward_epi(p, epi, ones, image, out, nslices, 0, e, st, bps);
     return rd_forward_epi(p, epi, ones, image, out, nslices, 0, e, st, bps);
 }
 
@@ -191,6 +193,8 @@ int projector_back_epi_any(const Plan& p, int epi, int ones, const float* sino, 
         return dd_back_epi(p, epi, ones, sino, x, nslices, 0, e, st, bps);
     if (p.projector == ITOMO_PROJ_PD)
         return pd_back_epi(p, epi, ones, sino, x, nslices, 0, e, st, bps);
+    if (p.projector == ITOMO_PROJ_SS)
+        return ss_back_epi(p, epi, ones, sino, x, nslices, 0, e, st, bps);
     Scratch s;
     int rc = get_scratch(p, nslices, &s, st);
     if (rc) return rc;

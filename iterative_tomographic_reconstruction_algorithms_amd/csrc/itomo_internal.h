@@ -68,6 +68,17 @@ struct PdView {
     double cabs, sdir, rsdir, dshift;
 };
 
+// Per-view constants of the slant-stacking projector (radon_ss.c:103-128).  Views with
+// |s| <= sqrt(1/2) step image rows u = y and sample columns q = (t - u s)/c; the others
+// step columns u = x and sample rows q = (t - u c)/s: q = (t - u a)/b.
+struct SsView {
+    float s, c;   // sin, cos as the reference computes them
+    float a, b;   // (s, c) or (c, s)
+    float rb;     // 1/b (FAST form of the division by b)
+    int ybranch;  // 1: step rows (|s| <= sqrt(1/2))
+    double rab;   // 1/|b| in double (radon_ss.c:128 `1.0/fabs(c)`)
+};
+
 struct Plan {
     int projector;   // ITOMO_PROJ_*
     int npix;
@@ -76,6 +87,8 @@ struct Plan {
     void* d_geom;    // device array of DdView / RdView
     void* h_geom;    // host copy
     float* d_lstar;  // distance-driven: [view][line] start of the dropped segment
+    int* d_range;    // slant-stacking: [view][bin] stepped range [u0, u1) (int2), clipped
+                     // to the image
     int fwd_var;     // distance-driven launch shapes (kFwdBins / kBackTiles index)
     int back_var;
     int method;      // pixel-driven: 0 nearest bin, 1 linear split (radon_pd.c:57-85)
@@ -122,12 +135,13 @@ inline size_t dd_back_blocks(int npix, int tile)
 inline size_t fwd_blocks_per_slice(const Plan& p)
 {
     if (p.projector == ITOMO_PROJ_DD) return dd_fwd_slots(p.npix, p.nang, kFwdBins[p.fwd_var]);
-    if (p.projector == ITOMO_PROJ_PD)  // partials come from the elementwise epilogue pass
+    if (p.projector == ITOMO_PROJ_PD || p.projector == ITOMO_PROJ_SS)
+        // partials come from the elementwise epilogue pass
         return ((size_t)p.nang * p.npix + kEpiChunk - 1) / kEpiChunk;
     return (size_t)((p.npix + kRdBlock - 1) / kRdBlock) * p.nang;
 }
 
-// Pixel-driven back projection: 64 x 4 pixels per block.
+// Pixel-driven and slant-stacking back projection: 64 x 4 pixels per block.
 inline size_t pd_back_blocks(int npix)
 {
     return (size_t)((npix + 63) / 64) * (size_t)((npix + 3) / 4);
@@ -137,7 +151,8 @@ inline size_t back_blocks_per_slice(const Plan& p)
 {
     const size_t n2 = (size_t)p.npix * p.npix;
     if (p.projector == ITOMO_PROJ_DD) return dd_back_blocks(p.npix, kBackTiles[p.back_var][0]);
-    if (p.projector == ITOMO_PROJ_PD) return pd_back_blocks(p.npix);
+    if (p.projector == ITOMO_PROJ_PD || p.projector == ITOMO_PROJ_SS)
+        return pd_back_blocks(p.npix);
     return (n2 + kEpiChunk - 1) / kEpiChunk;
 }
 
@@ -169,6 +184,10 @@ int rd_back(const Plan& p, const float* sino, float* image, int nslices,
 int pd_forward(const Plan& p, const float* image, float* sino, int nslices,
                int64_t img_stride, int64_t sino_stride, int accumulate, hipStream_t st);
 int pd_back(const Plan& p, const float* sino, float* image, int nslices,
+            int64_t sino_stride, int64_t img_stride, int accumulate, hipStream_t st);
+int ss_forward(const Plan& p, const float* image, float* sino, int nslices,
+               int64_t img_stride, int64_t sino_stride, int accumulate, hipStream_t st);
+int ss_back(const Plan& p, const float* sino, float* image, int nslices,
             int64_t sino_stride, int64_t img_stride, int accumulate, hipStream_t st);
 
 }  // namespace itomo

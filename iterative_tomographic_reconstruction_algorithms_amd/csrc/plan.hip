@@ -128,6 +128,87 @@ static void make_pd_view(float theta, int arith, PdView* g)
     g->dshift = 0.25 * std::fabs(g->cabs - g->sdir);
 }
 
+static const double kSsEps = 1e-5;                   // radon_ss.c:11
+static const double kSsS45 = 0.70710678118654757;    // radon_ss.c:12
+
+static void make_ss_view(float theta, int arith, SsView* g)
+{
+    float s, c;
+    view_trig(theta, arith, &s, &c);
+    std::memset(g, 0, sizeof *g);
+    g->s = s;
+    g->c = c;
+    g->ybranch = kSsS45 < std::fabs((double)s) ? 0 : 1;
+    g->a = g->ybranch ? s : c;
+    g->b = g->ybranch ? c : s;
+    g->rb = 1.0f / g->b;
+    g->rab = 1.0 / std::fabs((double)g->b);
+}
+
+static int ss_round_host(float v, int arith)
+{
+    if (arith == ITOMO_ARITH_FAST) {  // the fast build's float expansion of round()
+        const float r = v + std::copysign(0.49999997f, v);
+        return (int)std::trunc(r);
+    }
+    return (int)std::round((double)v);
+}
+
+// Middle two of {p, q, lo, hi} (lo <= hi), ascending: the reference sorts the four
+// values and keeps elements 1 and 2 (radon_ss.c:76-84).
+static void ss_middle_two(float p, float q, float lo, float hi, float* m, float* n)
+{
+    const float mn = p < q ? p : q, mx = p < q ? q : p;
+    const float m1 = mn > lo ? mn : lo, m2 = mx < hi ? mx : hi;
+    *m = m1 < m2 ? m1 : m2;
+    *n = m1 < m2 ? m2 : m1;
+}
+
+// Stepped range of every bin of one view (ray_limits, radon_ss.c:43-88, and the rounded
+// loop bounds of radon_ss.c:121,159), clipped to the image [-nh, npix-nh).
+static void ss_view_ranges(const SsView& g, int arith, int npix, int* out)
+{
+#pragma clang fp contract(off)
+    const int nh = (int)(npix * 0.5);
+    const float s = g.s, c = g.c;
+    const float nhf = (float)nh, nh1 = (float)(nh - 1), mnh = (float)(-nh);
+    for (int k = 0; k < npix; ++k) {
+        int u0 = 0, u1 = 0;
+        if (k < 2 * nh) {
+            const int t = k - nh;
+            const float tf = (float)t;
+            float lim[4];
+            if (std::fabs((double)s) < kSsEps) {
+                lim[0] = 0.0f; lim[1] = 0.0f; lim[2] = mnh; lim[3] = nh1;
+            } else if (std::fabs((double)c) < kSsEps) {
+                lim[0] = mnh; lim[1] = nh1; lim[2] = 0.0f; lim[3] = 0.0f;
+            } else {
+                float x1, x2, y1, y2;
+                if (arith == ITOMO_ARITH_FAST) {
+                    x1 = std::fma(nhf, s, tf) / c;
+                    x2 = std::fma(-s, nh1, tf) / c;
+                    y1 = std::fma(nhf, c, tf) / s;
+                    y2 = std::fma(-c, nh1, tf) / s;
+                } else {
+                    x1 = tf / c + (nhf * s) / c;
+                    x2 = tf / c - (nh1 * s) / c;
+                    y1 = tf / s + (nhf * c) / s;
+                    y2 = tf / s - (nh1 * c) / s;
+                }
+                ss_middle_two(x1, x2, mnh, nh1, &lim[0], &lim[1]);
+                ss_middle_two(y1, y2, mnh, nh1, &lim[2], &lim[3]);
+            }
+            u0 = ss_round_host(g.ybranch ? lim[2] : lim[0], arith);
+            u1 = ss_round_host(g.ybranch ? lim[3] : lim[1], arith);
+            u0 = u0 < -nh ? -nh : u0;
+            u1 = u1 > npix - nh ? npix - nh : u1;
+            if (u1 < u0) u1 = u0;
+        }
+        out[2 * k] = u0;
+        out[2 * k + 1] = u1;
+    }
+}
+
 int plan_scratch(const Plan& p, size_t bytes, void** ptr, hipStream_t st)
 {
     if (bytes > p.scratch_bytes) {
@@ -186,7 +267,8 @@ static int plan_create(itomo_plan** out, int projector, int npix, const float* a
 {
     if (!out) return set_error(ITOMO_EINVAL, "plan_create: null output pointer");
     *out = nullptr;
-    if (projector != ITOMO_PROJ_DD && projector != ITOMO_PROJ_RD && projector != ITOMO_PROJ_PD)
+    if (projector != ITOMO_PROJ_DD && projector != ITOMO_PROJ_RD &&
+        projector != ITOMO_PROJ_PD && projector != ITOMO_PROJ_SS)
         return set_error(ITOMO_EINVAL, "plan_create: unknown projector %d", projector);
     if (method != 0 && method != 1)
         return set_error(ITOMO_EINVAL, "plan_create: method %d is neither 0 (nearest) nor 1 "
@@ -218,6 +300,12 @@ static int plan_create(itomo_plan** out, int projector, int npix, const float* a
         for (int v = 0; v < nang && st == ITOMO_OK; ++v)
             st = make_dd_view(angles[v], arith, nang - 1 - v, &h[v]);
         p->h_geom = h;
+    } else if (projector == ITOMO_PROJ_SS) {
+        bytes = sizeof(SsView) * (size_t)nang;
+        SsView* h = (SsView*)std::malloc(bytes);
+        if (!h) { delete p; return set_error(ITOMO_ENOMEM, "plan_create: out of host memory"); }
+        for (int v = 0; v < nang; ++v) make_ss_view(angles[v], arith, &h[v]);
+        p->h_geom = h;
     } else if (projector == ITOMO_PROJ_PD) {
         bytes = sizeof(PdView) * (size_t)nang;
         PdView* h = (PdView*)std::malloc(bytes);
@@ -242,7 +330,27 @@ static int plan_create(itomo_plan** out, int projector, int npix, const float* a
         st = check_hip(hipMemcpy(p->d_geom, p->h_geom, bytes, hipMemcpyHostToDevice),
                        "plan_create: hipMemcpy");
     if (st == ITOMO_OK && projector == ITOMO_PROJ_DD) st = dd_build_line_ends(*p);
+    if (st == ITOMO_OK && projector == ITOMO_PROJ_SS) {
+        const size_t n = (size_t)nang * npix * 2;
+        std::vector<int> tab;
+        try {
+            tab.resize(n);
+        } catch (const std::bad_alloc&) {
+            st = set_error(ITOMO_ENOMEM, "plan_create: out of host memory");
+        }
+        if (st == ITOMO_OK) {
+            const SsView* g = (const SsView*)p->h_geom;
+            for (int v = 0; v < nang; ++v)
+                ss_view_ranges(g[v], arith, npix, tab.data() + (size_t)v * npix * 2);
+            st = check_hip(hipMalloc(&p->d_range, n * sizeof(int)), "plan_create: hipMalloc");
+        }
+        if (st == ITOMO_OK)
+            st = check_hip(hipMemcpy(p->d_range, tab.data(), n * sizeof(int),
+                                     hipMemcpyHostToDevice),
+                           "plan_create: hipMemcpy");
+    }
     if (st != ITOMO_OK) {
+        if (p->d_range) (void)hipFree(p->d_range);
         if (p->d_lstar) (void)hipFree(p->d_lstar);
         if (p->d_geom) (void)hipFree(p->d_geom);
         std::free(p->h_geom);
@@ -265,6 +373,12 @@ int itomo_plan_create_pd(itomo_plan** out, int npix, const float* angles, int na
     return plan_create(out, ITOMO_PROJ_PD, npix, angles, nang, method, arith);
 }
 
+int itomo_plan_create_ss(itomo_plan** out, int npix, const float* angles, int nang,
+                         int method, int arith)
+{
+    return plan_create(out, ITOMO_PROJ_SS, npix, angles, nang, method, arith);
+}
+
 int itomo_plan_destroy(itomo_plan* p)
 {
     if (!p) return ITOMO_OK;
@@ -272,6 +386,10 @@ int itomo_plan_destroy(itomo_plan* p)
     if (p->d_geom) st = check_hip(hipFree(p->d_geom), "plan_destroy: hipFree");
     if (p->d_lstar) {
         int st2 = check_hip(hipFree(p->d_lstar), "plan_destroy: hipFree");
+        if (!st) st = st2;
+    }
+    if (p->d_range) {
+        int st2 = check_hip(hipFree(p->d_range), "plan_destroy: hipFree");
         if (!st) st = st2;
     }
     if (p->scratch) {
@@ -291,6 +409,9 @@ int itomo_plan_trig(const itomo_plan* p, float* s_host, float* c_host)
         for (int v = 0; v < p->nang; ++v) { s_host[v] = g[v].s; c_host[v] = g[v].c; }
     } else if (p->projector == ITOMO_PROJ_PD) {
         const PdView* g = (const PdView*)p->h_geom;
+        for (int v = 0; v < p->nang; ++v) { s_host[v] = g[v].s; c_host[v] = g[v].c; }
+    } else if (p->projector == ITOMO_PROJ_SS) {
+        const SsView* g = (const SsView*)p->h_geom;
         for (int v = 0; v < p->nang; ++v) { s_host[v] = g[v].s; c_host[v] = g[v].c; }
     } else {
         const DdView* g = (const DdView*)p->h_geom;
@@ -342,6 +463,8 @@ int itomo_forward(const itomo_plan* p, const float* image, float* sino, int nsli
         return dd_forward(*p, image, sino, nslices, is, ss, accumulate, s);
     if (p->projector == ITOMO_PROJ_PD)
         return pd_forward(*p, image, sino, nslices, is, ss, accumulate, s);
+    if (p->projector == ITOMO_PROJ_SS)
+        return ss_forward(*p, image, sino, nslices, is, ss, accumulate, s);
     return rd_forward(*p, image, sino, nslices, is, ss, accumulate, s);
 }
 
@@ -356,6 +479,8 @@ int itomo_back(const itomo_plan* p, const float* sino, float* image, int nslices
         return dd_back(*p, sino, image, nslices, ss, is, accumulate, s);
     if (p->projector == ITOMO_PROJ_PD)
         return pd_back(*p, sino, image, nslices, ss, is, accumulate, s);
+    if (p->projector == ITOMO_PROJ_SS)
+        return ss_back(*p, sino, image, nslices, ss, is, accumulate, s);
     return rd_back(*p, sino, image, nslices, ss, is, accumulate, s);
 }
 
@@ -391,6 +516,12 @@ int itomo_radon_pd(float* image, int npix, const float* angles, int nang, int op
                    float* sino, void* stream)
 {
     return one_shot(ITOMO_PROJ_PD, image, npix, angles, nang, oper, method, sino, stream);
+}
+
+int itomo_radon_ss(float* image, int npix, const float* angles, int nang, int oper, int method,
+                   float* sino, void* stream)
+{
+    return one_shot(ITOMO_PROJ_SS, image, npix, angles, nang, oper, method, sino, stream);
 }
 
 }  // extern "C"

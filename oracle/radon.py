@@ -7,6 +7,7 @@ The wrappers reproduce what the reference's Cython modules do around the C call:
 * ``radon_ray_driven.pyx:15-61``: the same without the fftshift.
 * ``radon_pixel_driven.pyx:15-73``: degrees -> radians by ``2*pi/180`` (sic) when
   ``max(angles) > 2*pi``, fftshift, zeroed output, ``radon_pd(..., method, ...)``.
+* ``radon_slant_stacking.pyx:15-73``: the same without the fftshift.
 
 Unlike the reference wrappers they never modify the caller's angle array in place.
 """
@@ -49,14 +50,15 @@ def lib():
             f = getattr(L, name)
             f.argtypes = [_f64p, ctypes.c_int, _f64p, ctypes.c_int, ctypes.c_int, _f64p]
             f.restype = ctypes.c_int
-        f = L.oracle_radon_pd
-        f.argtypes = [_f32p, ctypes.c_int, _f32p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
-                      _f32p, ctypes.c_int]
-        f.restype = ctypes.c_int
-        f = L.oracle_radon_pd_f64
-        f.argtypes = [_f64p, ctypes.c_int, _f64p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
-                      _f64p]
-        f.restype = ctypes.c_int
+        for kind in ("pd", "ss"):
+            f = getattr(L, "oracle_radon_%s" % kind)
+            f.argtypes = [_f32p, ctypes.c_int, _f32p, ctypes.c_int, ctypes.c_int,
+                          ctypes.c_int, _f32p, ctypes.c_int]
+            f.restype = ctypes.c_int
+            f = getattr(L, "oracle_radon_%s_f64" % kind)
+            f.argtypes = [_f64p, ctypes.c_int, _f64p, ctypes.c_int, ctypes.c_int,
+                          ctypes.c_int, _f64p]
+            f.restype = ctypes.c_int
         _lib = L
     return _lib
 
@@ -120,9 +122,11 @@ def rd_back(sino, angles, mode=FAST):
     return _call("rd", sino, angles, 1, mode, shift=False)
 
 
-def _call_pd(x, angles, oper, mode, method, init=None):
+def _call_pd(x, angles, oper, mode, method, init=None, kind="pd"):
     L = lib()
-    a32 = _prep_angles(angles, True, 2 * np.pi / 180.0)   # radon_pixel_driven.pyx:27-30
+    # radon_pixel_driven.pyx:27-30 / radon_slant_stacking.pyx:27-30: degrees times 2*pi/180,
+    # fftshift for pd only (commented out in the slant-stacking wrapper)
+    a32 = _prep_angles(angles, kind == "pd", 2 * np.pi / 180.0)
     nang = a32.size
     npix = x.shape[1]
     dt = np.float64 if mode == F64 else np.float32
@@ -136,12 +140,13 @@ def _call_pd(x, angles, oper, mode, method, init=None):
     else:
         args = (out, npix, None, nang, 1, int(method), src)
     if mode == F64:
-        rc = L.oracle_radon_pd_f64(args[0], npix, a32.astype(np.float64), nang, oper,
-                                   int(method), args[6])
+        rc = getattr(L, "oracle_radon_%s_f64" % kind)(args[0], npix, a32.astype(np.float64),
+                                                      nang, oper, int(method), args[6])
     else:
-        rc = L.oracle_radon_pd(args[0], npix, a32, nang, oper, int(method), args[6], mode)
+        rc = getattr(L, "oracle_radon_%s" % kind)(args[0], npix, a32, nang, oper,
+                                                  int(method), args[6], mode)
     if rc != 0:
-        raise MemoryError("oracle_radon_pd failed")
+        raise MemoryError("oracle_radon_%s failed" % kind)
     return out
 
 
@@ -156,9 +161,38 @@ def pd_back(sino, angles, mode=FAST, method=1, init=None):
     return _call_pd(sino, angles, 1, mode, method, init)
 
 
+def ss_forward(image, angles, mode=FAST, method=1, init=None):
+    """Slant-stacking forward projection, (N,N) -> (V,N); sinogram row V-1-v holds view
+    v (radon_ss.c:111); no fftshift.  The facade passes method 1
+    (class_projectors_radon.py:68)."""
+    return _call_pd(image, angles, 0, mode, method, init, kind="ss")
+
+
+def ss_back(sino, angles, mode=FAST, method=1, init=None):
+    return _call_pd(sino, angles, 1, mode, method, init, kind="ss")
+
+
+def ss_edge_bins(npix, c_angles, method=1, mode=FAST):
+    """(flags, (lo, hi)) for the slant-stacking C receiving ``c_angles`` (already
+    converted to radians): flags[row, bin] = 1 where the reference steps or samples
+    outside the image square, (lo, hi) = the flat image-index extent it reaches."""
+    a = np.ascontiguousarray(c_angles, dtype=np.float32)
+    flags = np.zeros((a.size, npix), dtype=np.uint8)
+    ext = np.zeros(2, dtype=np.int64)
+    L = lib()
+    f = L.oracle_ss_edge_bins
+    f.argtypes = [ctypes.c_int, _f32p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                  np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS"),
+                  np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")]
+    f.restype = ctypes.c_int
+    f(npix, a, a.size, int(method), 1 if mode == FAST else 0, flags, ext)
+    return flags.astype(bool), (int(ext[0]), int(ext[1]))
+
+
 def forward(oper, image, angles, mode=FAST):
-    return {"dd": dd_forward, "rd": rd_forward, "pd": pd_forward}[oper](image, angles, mode)
+    return {"dd": dd_forward, "rd": rd_forward, "pd": pd_forward,
+            "ss": ss_forward}[oper](image, angles, mode)
 
 
 def back(oper, sino, angles, mode=FAST):
-    return {"dd": dd_back, "rd": rd_back, "pd": pd_back}[oper](sino, angles, mode)
+    return {"dd": dd_back, "rd": rd_back, "pd": pd_back, "ss": ss_back}[oper](sino, angles, mode)

@@ -54,7 +54,7 @@ def _angles(angles, proj):
         # radon_distance_driven.pyx:25-26 (pd/ss multiply by 2*pi/180 instead:
         # radon_pixel_driven.pyx:27-28)
         a *= np.float32((2 * np.pi if proj in ("pd", "ss") else np.pi) / 180.0)
-    if proj in ("dd", "pd", "ss"):
+    if proj in ("dd", "pd"):      # radon_slant_stacking.pyx:30 has the fftshift commented out
         a = np.fft.fftshift(a)
     return np.ascontiguousarray(a, dtype=np.float32)
 
@@ -67,12 +67,40 @@ def _angles(angles, proj):
 _GUARD = 1
 
 
+def _ss_image_guard(npix, a, flags, method):
+    """Zero rows to put before / after the image so that every flat index the
+    slant-stacking C reaches for these angles stays inside the buffer (for lines that
+    miss the image square it indexes rows outside the image, radon_ss.c:121-191)."""
+    from . import radon
+    _, (lo, hi) = radon.ss_edge_bins(npix, a, method,
+                                     radon.FAST if flags == "fast" else radon.IEEE)
+    before = max(0, -(lo // npix)) + 1
+    after = max(0, hi // npix - (npix - 1)) + 1
+    return before, after
+
+
+def _ss_call(f, x, a, oper, method, flags):
+    npix = x.shape[1]
+    before, after = _ss_image_guard(npix, a, flags, method)
+    buf = np.zeros((before + npix + after, npix), dtype=np.float32)
+    img = buf[before:before + npix]
+    if oper == 0:
+        img[:] = x
+        sino = np.zeros((a.size, npix), dtype=np.float32)
+        f(img, npix, a, a.size, 0, method, sino)
+        return sino
+    f(img, npix, a, a.size, 1, method, np.ascontiguousarray(x, dtype=np.float32))
+    return img.copy()
+
+
 def forward(proj, image, angles, flags="fast", method=1):
     f = _fn(proj, flags)
     img = np.ascontiguousarray(image, dtype=np.float32)
     npix = img.shape[0]
     a = _angles(angles, proj)
-    if proj in ("pd", "ss"):
+    if proj == "ss":
+        return _ss_call(f, img, a, 0, method, flags)
+    if proj == "pd":
         g = np.zeros((a.size + 2 * _GUARD, npix), dtype=np.float32)
         f(img, npix, a, a.size, 0, method, g[_GUARD:])
         return g[_GUARD:_GUARD + a.size].copy()
@@ -86,8 +114,10 @@ def back(proj, sino, angles, flags="fast", method=1):
     s = np.ascontiguousarray(sino, dtype=np.float32)
     nang, npix = s.shape
     a = _angles(angles, proj)
+    if proj == "ss":
+        return _ss_call(f, s, a, 1, method, flags)
     image = np.zeros((npix, npix), dtype=np.float32)
-    if proj in ("pd", "ss"):
+    if proj == "pd":
         g = np.zeros((nang + 2 * _GUARD, npix), dtype=np.float32)
         g[_GUARD:_GUARD + nang] = s
         f(image, npix, a, nang, 1, method, g[_GUARD:])

@@ -1,7 +1,8 @@
 """Write the golden projector vectors in tests/golden/radon_ref.npz.
 
 Inputs are seeded phantoms / sinograms; expected outputs come from the reference's own
-radon C (radon_dd.c, radon_rd.c; radon_pd.c into radon_pd_ref.npz) compiled from the reference sources by
+radon C (radon_dd.c, radon_rd.c; radon_pd.c and radon_ss.c into radon_pd_ref.npz and
+radon_ss_ref.npz) compiled from the reference sources by
 ``make -C oracle ref`` into ``oracle/_ref`` -- both with the reference's flags
 (``fast``: -O3 -ffast-math) and with IEEE flags (``strict``).  The angle arrays stored
 are what the reference facade passes to the Cython wrappers (float32 radians,
@@ -24,6 +25,7 @@ from iterative_tomographic_reconstruction_algorithms_amd.phantom import (  # noq
 
 OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "radon_ref.npz")
 OUT_PD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "radon_pd_ref.npz")
+OUT_SS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "radon_ss_ref.npz")
 
 
 def facade_angles(deg):
@@ -80,6 +82,7 @@ def main():
     np.savez_compressed(OUT, **data)
     print("wrote %s (%d arrays, %.1f MB)" % (OUT, len(data), os.path.getsize(OUT) / 1e6))
     pixel_driven()
+    slant_stacking()
 
 
 def pd_cases():
@@ -121,6 +124,44 @@ def pixel_driven():
     np.savez_compressed(OUT_PD, **data)
     print("wrote %s (%d arrays, %.1f MB)" % (OUT_PD, len(data),
                                              os.path.getsize(OUT_PD) / 1e6))
+
+
+
+def slant_stacking():
+    """radon_ss.c with method 0 and 1 (the facade's), through the wrapper's angle handling
+    (2*pi/180 degree factor, no fftshift: radon_slant_stacking.pyx:27-30).  For lines that
+    miss the image square the reference indexes outside the image rows; oracle.ref runs
+    it on an image padded with zero rows so that every access stays in the buffer, and
+    the fixture stores which bins those are ("edge", from the oracle): forward values
+    there are the reference's out-of-image reads and are not compared, and the back
+    projection input is zero there."""
+    from oracle import radon as O
+    if not ref.available("ss"):
+        sys.exit("oracle/_ref is not built: run `make -C oracle ref` first")
+    data = {}
+    for k, (name, n, ang) in enumerate(pd_cases()):
+        rng = np.random.default_rng(600 + k)
+        img = (ellipses(n, 10, seed=700 + k) if k % 2 == 0
+               else (rng.random((n, n)) - 0.25).astype(np.float32))
+        c_ang = np.array(ang, np.float32)
+        if c_ang.size and c_ang.max() > 2 * np.pi:
+            c_ang *= np.float32(2 * np.pi / 180.0)
+        for method in (0, 1):
+            key = "ss_%s_%d_%d_m%d" % (name, n, ang.size, method)
+            edge = np.zeros((ang.size, n), bool)
+            for flags, mode in (("fast", O.FAST), ("strict", O.IEEE)):
+                edge |= O.ss_edge_bins(n, c_ang, method, mode)[0]
+            sino = np.where(edge, 0, rng.random((ang.size, n))).astype(np.float32)
+            data[key + "/angles"] = ang
+            data[key + "/image"] = img
+            data[key + "/sino"] = sino
+            data[key + "/edge"] = edge
+            for flags in ("fast", "strict"):
+                data[key + "/fwd_" + flags] = ref.forward("ss", img, ang, flags, method)
+                data[key + "/back_" + flags] = ref.back("ss", sino, ang, flags, method)
+    np.savez_compressed(OUT_SS, **data)
+    print("wrote %s (%d arrays, %.1f MB)" % (OUT_SS, len(data),
+                                             os.path.getsize(OUT_SS) / 1e6))
 
 
 if __name__ == "__main__":

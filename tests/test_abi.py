@@ -64,3 +64,5 @@ def test_plan_rejects_bad_arguments_without_touching_the_gpu():
     assert rc == -1 and b"method" in L.itomo_last_error()
     rc = L.itomo_radon_pd(None, 16, ctypes.cast(ang, ctypes.c_void_p), 2, 2, 0, None, None)
     assert rc == -1
+    rc = L.itomo_plan_create_ss(ctypes.byref(h), 16, ctypes.cast(ang, ctypes.c_void_p), 2, -1, 0)
+    assert rc == -1 and b"method" in L.itomo_last_error()

@@ -1,7 +1,7 @@
 """GPU parity of the fused SIRT / MLEM steps against the numpy restatement of
 art.py:sirt / sir.py:em driven by the oracle projectors.
 
-Distance- and pixel-driven: the GPU iterates are bitwise equal to the float32 restatement (the
+Distance-driven, pixel-driven and slant-stacking: the GPU iterates are bitwise equal to the float32 restatement (the
 projectors are, and the fused epilogues perform the same float32 operations in the
 same order).  Ray-driven: the scatter back projection reorders float additions, so the
 iterate after k steps is held to relative L2 <= 1e-5.  FP64 monitors are compared to
@@ -34,13 +34,14 @@ def setup(proj, n, v, seed=1):
 
 
 @pytest.mark.parametrize("proj,n,v,iters", [("dd", 64, 36, 6), ("dd", 96, 75, 4),
-                                            ("rd", 64, 36, 4), ("pd", 64, 36, 6)])
+                                            ("rd", 64, 36, 4), ("pd", 64, 36, 6),
+                                            ("ss", 64, 36, 6)])
 def test_sirt_matches_restatement(proj, n, v, iters):
     plan, img, A, At = setup(proj, n, v)
     b_np = A(img)
     R, C = plan.sirt_weights()
     R_ref, C_ref = S.sirt_weights(A, At, n, v)
-    if proj in ("dd", "pd"):
+    if proj in ("dd", "pd", "ss"):
         np.testing.assert_array_equal(R.cpu().numpy(), R_ref)
         np.testing.assert_array_equal(C.cpu().numpy(), C_ref)
     else:
@@ -55,7 +56,7 @@ def test_sirt_matches_restatement(proj, n, v, iters):
         plan.sirt_iterate(x, b, R, C, work, pc_beta=0.0, stats=stats)
         res.append(float(stats[0].sqrt()))
     x_ref, info, resid = S.sirt(np.zeros((n, n), np.float32), b_np, A, At, iters, pc=0.0)
-    if proj in ("dd", "pd"):
+    if proj in ("dd", "pd", "ss"):
         np.testing.assert_array_equal(x.cpu().numpy(), x_ref)
         np.testing.assert_allclose(res, resid, rtol=1e-12)
     else:
@@ -79,7 +80,7 @@ def test_sirt_split_halves_equal_iterate():
 
 
 @pytest.mark.parametrize("proj,n,v,iters", [("dd", 64, 36, 5), ("rd", 48, 30, 3),
-                                            ("pd", 48, 30, 4)])
+                                            ("pd", 48, 30, 4), ("ss", 48, 30, 4)])
 def test_mlem_matches_restatement(proj, n, v, iters):
     plan, img, A, At = setup(proj, n, v, seed=4)
     rng = np.random.default_rng(0)
@@ -96,7 +97,7 @@ def test_mlem_matches_restatement(proj, n, v, iters):
         plan.mlem_iterate(x, b, inv_d, work, stats=stats)
         errs.append(float(stats[0]) / bsq)
     x_ref, err = S.mlem(np.ones((n, n), np.float32), b_np, A, At, iters)
-    if proj in ("dd", "pd"):
+    if proj in ("dd", "pd", "ss"):
         np.testing.assert_array_equal(x.cpu().numpy(), x_ref)
         np.testing.assert_allclose(errs, err, rtol=1e-12)
     else:
