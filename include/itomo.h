@@ -219,6 +219,37 @@ int itomo_sps_iterate(const itomo_plan *plan, float *x, const float *b, const fl
                       void *stream);
 
 /* ---------------------------------------------------------------------------------
+ * SPS penalties (sir.py:455-560 over the reference's penalty module).
+ * itomo_penalty replaces penalty.pyx:16-106 (huber_num/_den over huber.c:66-118,
+ * tikhonov_num/_den over tikhonov.c:162-195, haar_num/_den over haar.c:242-275): for each
+ * of nslices nr x nc images, r_num[i][j] = sum over the 3x3 neighbours inside the image of
+ * w_k * phi'(I[i][j] - I[nb]) and r_den the curvature term (huber: 2 w_k / (1 + |d/delta|);
+ * tikhonov: the constant 2; haar: 0), neighbour weights 1/sqrt(2), 1, 4
+ * (penalty_weights.c:7-15), accumulated in the reference's order.  arith FAST = the
+ * module's -ffast-math build (create_penalty_module.py), IEEE = the C as written.
+ * itomo_sobel_magnitude: sqrt(gx^2 + gy^2) of scipy.ndimage.sobel(x, axis, mode='reflect')
+ * for both axes (sir.py:339-343), the image the Huber penalty is applied to.
+ * itomo_sps_step: one SPS iteration with the reference's options (sir.py:428-540):
+ *   work = (A x - b) w; num = A^T work;
+ *   x <- x - (num + beta R_num) / (den + beta R_den + 2^-52), quotient in FP64, with R from
+ *   x (tikhonov, haar) or from sobel_magnitude(x) (huber, delta), penalty NONE: beta unused;
+ *   clip != 0: x <- np.clip(x, 0, max x) over all slices (sir.py:539-540, the reference's
+ *   default when there is no edge padding).
+ * img_work: nslices x npix x npix floats (penalty NONE) or 4x that (with a penalty).
+ * stats as for itomo_sps_iterate; [1] uses the clipped iterate.
+ * ------------------------------------------------------------------------------- */
+#define ITOMO_PENALTY_NONE 0
+#define ITOMO_PENALTY_HUBER 1
+#define ITOMO_PENALTY_TIKHONOV 2
+#define ITOMO_PENALTY_HAAR 3
+int itomo_penalty(const float *image, int nr, int nc, int nslices, int kind, float delta,
+                  int arith, float *r_num, float *r_den, void *stream);
+int itomo_sobel_magnitude(const float *x, int nr, int nc, int nslices, float *g, void *stream);
+int itomo_sps_step(const itomo_plan *plan, float *x, const float *b, const float *w,
+                   const float *den, float *work, float *img_work, int nslices, int penalty,
+                   float beta, float delta, int clip, double *stats, void *stream);
+
+/* ---------------------------------------------------------------------------------
  * Conjugate gradients on the normal equations (algorithms/cg.py:83-176) and the
  * ADMM-TV inner solve (cg.py:256-310, admm.py:470-646), float32 vectors with FP64
  * inner products and step sizes kept on the device.

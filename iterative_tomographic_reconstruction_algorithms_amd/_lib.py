@@ -12,6 +12,7 @@ LIB_PATH = os.path.join(HERE, "libitomo.so")
 ITOMO_OK = 0
 PROJ = {"dd": 0, "rd": 1, "pd": 2, "ss": 3}
 ARITH = {"fast": 0, "ieee": 1}
+PENALTY = {None: 0, "huber": 1, "tikhonov": 2, "haar": 3}
 DOT_WORK = 3072
 
 
@@ -51,6 +52,9 @@ _SIGS = {
     "itomo_mlem_back": ([_vp, _vp, _vp, _vp, _i, _vp, _vp], _i),
     "itomo_mlem_iterate": ([_vp, _vp, _vp, _vp, _vp, _i, _vp, _vp], _i),
     "itomo_dot_f64": ([_vp, _vp, _i64, _vp, _vp, _vp], _i),
+    "itomo_penalty": ([_vp, _i, _i, _i, _i, _f, _i, _vp, _vp, _vp], _i),
+    "itomo_sobel_magnitude": ([_vp, _i, _i, _i, _vp, _vp], _i),
+    "itomo_sps_step": ([_vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _f, _f, _i, _vp, _vp], _i),
     "itomo_unet_conv3x3": ([_vp, _i, _i, _i, _i, _vp, _vp, _i, _vp, _i, _vp, _vp], _i),
     "itomo_unet_flip": ([_vp, _i, _i, _vp, _vp], _i),
     "itomo_unet_wgrad": ([_vp, _vp, _i, _i, _i, _i, _i, _vp, _vp, _vp], _i),

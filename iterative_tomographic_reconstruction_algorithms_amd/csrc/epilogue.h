@@ -204,6 +204,12 @@ int dd_forward_epi(const Plan& p, int epi, int ones_input, const float* image, f
 int dd_back_epi(const Plan& p, int epi, int ones_input, const float* sino, float* out,
                 int nslices, int accumulate, const EpiArgs& e, hipStream_t st,
                 int* blocks_per_slice);
+// any projector, with an epilogue (iter.hip)
+int projector_forward_epi_any(const Plan& p, int epi, int ones, const float* image, float* out,
+                              int nslices, const EpiArgs& e, hipStream_t st, int* bps);
+int projector_back_epi_any(const Plan& p, int epi, int ones, const float* sino, float* x,
+                           int nslices, const EpiArgs& e, hipStream_t st, int* bps);
+
 // pd.hip
 int pd_forward_epi(const Plan& p, int epi, int ones_input, const float* image, float* out,
                    int nslices, int accumulate, const EpiArgs& e, hipStream_t st,

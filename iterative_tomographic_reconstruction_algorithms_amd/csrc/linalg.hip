@@ -296,7 +296,9 @@ sps_w_kernel(const float* __restrict__ b, float* __restrict__ w, float* __restri
     for (int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x; k < n;
          k += (int64_t)gridDim.x * 256) {
         const float a = a1[k % per];  // read before w[k] is written: a1 may alias w
-        const float wv = expf(-__fdiv_rn(b[k], bm));
+        // exp in double rounded to float: numpy's float32 exp (the reference) is not
+        // correctly rounded either; this is, to within the double exp's error
+        const float wv = (float)exp(-(double)__fdiv_rn(b[k], bm));
         w[k] = wv;
         aux[k] = __fmul_rn(wv, a);
     }

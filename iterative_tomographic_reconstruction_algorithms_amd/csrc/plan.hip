@@ -235,7 +235,9 @@ int get_scratch(const Plan& p, int nslices, Scratch* s, hipStream_t st)
     const size_t ns = (size_t)nslices, n2 = (size_t)p.npix * p.npix;
     const size_t nxt = n2 * ns;
     const size_t nf = fwd_blocks_per_slice(p) * 3 * ns;
-    const size_t nb = back_blocks_per_slice(p) * 3 * ns;
+    // back partials also serve the elementwise SPS update (one slot per kEpiChunk pixels)
+    const size_t nbb = back_blocks_per_slice(p), nbe = (n2 + kEpiChunk - 1) / kEpiChunk;
+    const size_t nb = (nbb > nbe ? nbb : nbe) * 3 * ns;
     const size_t ni = p.projector == ITOMO_PROJ_RD ? n2 * ns : 0;
     auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
     const size_t o_pf = up(nxt * sizeof(float));

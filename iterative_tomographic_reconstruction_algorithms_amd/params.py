@@ -13,8 +13,9 @@ class param:
         npix_op=None, nang=None, nz=1, n_iter=None, eps=None, pc=-1.0, tv=0.0,
         mask=None, mask_add=None, edge_padding=0.0, index_start=0, index_end=None,
         plot=False, checkit=False, projector="dist-driv", algorithm="sirt",
-        num_cores=-1, reg=None, reg_cost=0.0, n_iter_pcg=4, lambd1=1.0, lambd2=1.0,
-        mu=1.0, init_object=False, radon_degree=0,
+        num_cores=-1, reg=None, reg_cost=0.1, huber_cost=0.1, n_iter_pcg=4, lambd1=1.0,
+        lambd2=1.0,
+        mu=1.0, init_object=False, radon_degree=0,     # reg_cost / huber_cost: sir.py:128,145
     )
 
     def __init__(self, **kw):

@@ -36,13 +36,21 @@ def em(x, b, tp, param):
 
 
 def sps(x, b, tp, param):
-    """sir.py:348-600 without penalty: returns (x, info) with info rows [err, 0]."""
-    if getattr(param, "reg", None) not in (None, "none") and getattr(param, "reg_cost", 0.0):
-        raise NotImplementedError("SPS penalties are outside the GPU path")
+    """sir.py:348-600: returns (x, info) with info rows [err, 0].  param.reg in (None,
+    'huber', 'tikhonov', 'haar') with param.reg_cost (beta) and param.huber_cost (delta);
+    the iterate is clipped to [0, max x] unless the problem is edge-padded
+    (sir.py:539-540)."""
+    reg = getattr(param, "reg", None)
+    reg = None if reg in (None, "none") else reg
+    if reg not in (None, "huber", "tikhonov", "haar"):
+        raise ValueError("unknown SPS penalty %r" % (reg,))
     n_iter = int(param.n_iter) if param.n_iter is not None and param.n_iter < 1e19 else 200
+    clip = not bool(getattr(param, "edge_padding", False))
     xd = _dev(x).clone()
     plan = tp.plan(xd.shape[-1])
-    xd, err = solvers.sps(plan, xd, _dev(b), n_iter)
+    xd, err = solvers.sps(plan, xd, _dev(b), n_iter, penalty=reg,
+                          beta=float(getattr(param, "reg_cost", 0.0) or 0.0),
+                          delta=float(getattr(param, "huber_cost", 0.1) or 0.1), clip=clip)
     xd = xd.reshape(x.shape) if isinstance(x, torch.Tensor) else xd.reshape(np.shape(x))
     return _out(xd, x), np.stack([err, np.zeros_like(err)], axis=1)
 

@@ -109,9 +109,13 @@ def mlem(plan, x, b, n_iter, eps=-1.0):
 
 # ---- SPS ------------------------------------------------------------------------------
 
-def sps(plan, x, b, n_iter):
-    """sir.py:348-600 without penalty (beta = 0): separable paraboloidal surrogate with
-    w = exp(-b/max b), den = A^T(w * A(1)), FP64 quotient in the update.
+def sps(plan, x, b, n_iter, penalty=None, beta=0.0, delta=0.1, clip=True):
+    """sir.py:348-560: separable paraboloidal surrogate with w = exp(-b/max b),
+    den = A^T(w * A(1)) and the FP64 quotient in the update
+    x <- x - (A^T(w (A x - b)) + beta R') / (den + beta R'' + eps).  ``penalty`` is None,
+    "huber" (on the Sobel magnitude of x, threshold ``delta``), "tikhonov" or "haar"
+    (sir.py:455-530); ``clip`` applies np.clip(x, 0, max x) after each step, the
+    reference's default when there is no edge padding (sir.py:539-540).
     Returns (x, err) with err[k] = sum ((A x_k - b) w)^2 / sum b^2 over all slices
     (sir.py:546)."""
     x = _as3(x)
@@ -123,13 +127,41 @@ def sps(plan, x, b, n_iter):
     bmax = torch.empty(1024, dtype=torch.float32, device=x.device)
     L.check(L.lib().itomo_sps_weights(plan._h, _p(b), nz, _p(w), _p(den), _p(work), _p(bmax),
                                       stream_handle()), "sps_weights")
+    kind = L.PENALTY[penalty]
+    img_work = torch.empty(((4 if kind else 1) * nz,) + tuple(x.shape[-2:]),
+                           dtype=torch.float32, device=x.device)
     hist = torch.zeros((n_iter, 3 * nz), dtype=torch.float64, device=x.device)
     for it in range(n_iter):
-        L.check(L.lib().itomo_sps_iterate(plan._h, _p(x), _p(b), _p(w), _p(den), _p(work), nz,
-                                          _p(hist[it]), stream_handle()), "sps_iterate")
+        L.check(L.lib().itomo_sps_step(plan._h, _p(x), _p(b), _p(w), _p(den), _p(work),
+                                       _p(img_work), nz, kind, float(beta), float(delta),
+                                       int(bool(clip)), _p(hist[it]), stream_handle()),
+                "sps_step")
     bden = float(torch.sum(b.double() ** 2))
     h = hist.cpu().numpy().reshape(n_iter, nz, 3)
     return x, h[:, :, 0].sum(axis=1) / bden
+
+
+def penalty(kind, image, delta=0.1, arith="fast"):
+    """(R_num, R_den) of the reference's penalty module (penalty.pyx:16-106) for an
+    (..., nr, nc) float32 GPU tensor."""
+    img = image.contiguous()
+    nr, nc = img.shape[-2:]
+    ns = img.numel() // (nr * nc)
+    rn = torch.empty_like(img)
+    rd = torch.empty_like(img)
+    L.check(L.lib().itomo_penalty(_p(img), nr, nc, ns, L.PENALTY[kind], float(delta),
+                                  L.ARITH[arith], _p(rn), _p(rd), stream_handle()), "penalty")
+    return rn, rd
+
+
+def sobel_magnitude(x):
+    """sir.py:339-343 on an (..., nr, nc) float32 GPU tensor."""
+    x = x.contiguous()
+    nr, nc = x.shape[-2:]
+    g = torch.empty_like(x)
+    L.check(L.lib().itomo_sobel_magnitude(_p(x), nr, nc, x.numel() // (nr * nc), _p(g),
+                                          stream_handle()), "sobel_magnitude")
+    return g
 
 
 # ---- finite differences -----------------------------------------------------------------

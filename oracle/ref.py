@@ -124,3 +124,26 @@ def back(proj, sino, angles, flags="fast", method=1):
     else:
         f(image, npix, a, nang, 1, s)
     return image
+
+
+def penalty(kind, image, delta=0.1, flags="fast"):
+    """(R_num, R_den) from the reference's huber.c / tikhonov.c / haar.c, called the way
+    penalty.pyx:16-106 calls them (fresh zero outputs; tikhonov den = 2, haar den = 0)."""
+    p = os.path.join(REF_DIR, "libpenalty_%s.so" % flags)
+    if not os.path.exists(p):
+        raise FileNotFoundError(p)
+    L = ctypes.CDLL(p)
+    img = np.ascontiguousarray(image, dtype=np.float32)
+    nr, nc = img.shape
+    rn = np.zeros((nr, nc), np.float32)
+    if kind == "huber":
+        L.huber.argtypes = [_f32p, ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_char_p,
+                            _f32p]
+        rd = np.zeros((nr, nc), np.float32)
+        L.huber(img, nr, nc, float(delta), b"nom", rn)
+        L.huber(img, nr, nc, float(delta), b"den", rd)
+        return rn, rd
+    f = getattr(L, kind)
+    f.argtypes = [_f32p, ctypes.c_int, ctypes.c_int, _f32p]
+    f(img, nr, nc, rn)
+    return rn, np.full((nr, nc), 2.0 if kind == "tikhonov" else 0.0, np.float32)

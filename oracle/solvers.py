@@ -67,23 +67,40 @@ def mlem(x0, b, A, At, n_iter, dtype=np.float32):
     return x, np.array(err)
 
 
-def sps(x0, b, A, At, n_iter, dtype=np.float32):
-    """sir.py:348-540 with beta = 0: w = exp(-b/max b); den = At(w*A(1));
-    x <- x - At(w*(A x - b)) / (den + spacing(1)), the quotient evaluated in float64
-    (the GPU path's FP64 denominator).  Returns (x, err) with err[k] = sum(w(Ax_k-b))^2/sum b^2."""
+def sps(x0, b, A, At, n_iter, dtype=np.float32, penalty=None, beta=0.0, delta=0.1,
+        clip=True):
+    """sir.py:348-560: w = exp(-b/max b); den = At(w*A(1));
+    x <- x - (At(w*(A x - b)) + beta R') / (den + beta R'' + spacing(1)), the quotient
+    evaluated in float64 (the GPU path's FP64 denominator), R from the restated penalty
+    module (oracle/penalty_oracle.c, the reference build's arithmetic) on x, or on the
+    scipy Sobel magnitude of x for "huber"; then np.clip(x, 0, max x) when ``clip``
+    (sir.py:539-540).  Returns (x, err) with err[k] = sum(w(Ax_k-b))^2/sum b^2."""
     n = x0.shape[-1]
     b = b.astype(dtype)
-    w = np.exp(-b / np.max(b)).astype(dtype)
+    # the reference's float32 np.exp is not correctly rounded; like the GPU path, take
+    # exp in double and round once (differs from numpy's by at most 1 ulp in w)
+    w = np.exp((-b / np.max(b)).astype(dtype).astype(np.float64)).astype(dtype)
     den = At((w * A(np.ones((n, n), dtype=dtype))).astype(dtype)).astype(dtype)
     x = x0.astype(dtype).copy()
     bden = np.sum(b.astype(np.float64) ** 2)
     err = []
     for _ in range(n_iter):
+        if penalty is not None:
+            from . import penalty as P
+            src = P.sobel_magnitude(x) if penalty == "huber" else x
+            rn, rd = P.penalty(penalty, src, delta)
         bd = ((A(x).astype(dtype) - b) * w).astype(dtype)
         err.append(np.sum(bd.astype(np.float64) ** 2) / bden)
         num = At(bd).astype(dtype)
-        q = num.astype(np.float64) / (den.astype(np.float64) + np.spacing(1))
-        x = (x.astype(np.float64) - q).astype(dtype)
+        p = num.astype(np.float64)
+        q = den.astype(np.float64)
+        if penalty is not None:     # beta is float32 in the reference (sir.py:128)
+            p = p + np.float64(np.float32(beta)) * rn.astype(np.float64)
+            q = q + np.float64(np.float32(beta)) * rd.astype(np.float64)
+        q = q + np.spacing(1)
+        x = (x.astype(np.float64) - p / q).astype(dtype)
+        if clip:
+            x = np.clip(x, 0, np.max(x))
     return x, np.array(err)
 
 
