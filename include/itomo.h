@@ -285,6 +285,16 @@ int itomo_tv_rhs(float *v, const float *u, const float *alpha, int64_t n, float 
  * (admm.py:460-461, 565-593, utils.py:214-216). */
 int itomo_admm_tv_update(const float *x, float *u, float *alpha, int npix, int nslices,
                          float mu, float lambd2, double *scal, double *work, void *stream);
+/* Lasso-L1 ADMM (admm.py:291-447 with cg.py:189-245): the inner CG solves
+ * (A^T A + (lambd1 + mu) I) x = A^T b + v with v = mu u - alpha (itomo_tv_rhs); its
+ * residual r = (atb + v) - (atax + c x), c = lambd1 + mu, scal[0] = <r, r>; its H p is
+ * itomo_cg_hp with g = NULL and lambd1 = c.  The multiplier step: u = soft(x + alpha/mu,
+ * lambd2/mu), alpha += x - u, scal[3] = sum (x - u)^2. */
+int itomo_l1_residual(float *r, const float *atb, const float *atax, const float *x,
+                      const float *v, int npix, int nslices, float c, double *scal,
+                      double *work, void *stream);
+int itomo_admm_l1_update(const float *x, float *u, float *alpha, int npix, int nslices,
+                         float mu, float lambd2, double *scal, double *work, void *stream);
 
 /* ---------------------------------------------------------------------------------
  * Sinogram filtering for filtered back projection (filters.py:35-113, dpc = False).

@@ -52,6 +52,8 @@ _SIGS = {
     "itomo_mlem_back": ([_vp, _vp, _vp, _vp, _i, _vp, _vp], _i),
     "itomo_mlem_iterate": ([_vp, _vp, _vp, _vp, _vp, _i, _vp, _vp], _i),
     "itomo_dot_f64": ([_vp, _vp, _i64, _vp, _vp, _vp], _i),
+    "itomo_l1_residual": ([_vp, _vp, _vp, _vp, _vp, _i, _i, _f, _vp, _vp, _vp], _i),
+    "itomo_admm_l1_update": ([_vp, _vp, _vp, _i, _i, _f, _f, _vp, _vp, _vp], _i),
     "itomo_penalty": ([_vp, _i, _i, _i, _i, _f, _i, _vp, _vp, _vp], _i),
     "itomo_sobel_magnitude": ([_vp, _i, _i, _i, _vp, _vp], _i),
     "itomo_sps_step": ([_vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _f, _f, _i, _vp, _vp], _i),

@@ -1,5 +1,5 @@
-"""Drop-in for the CG and ADMM-TV solvers of the reference's algorithms/admm.py
-(admm.py:245-278 conj_grad, 470-646 lasso_tv, 878-953 admm) on the GPU."""
+"""Drop-in for the CG, ADMM-L1 and ADMM-TV solvers of the reference's algorithms/admm.py
+(admm.py:245-278 conj_grad, 291-447 lasso_l1, 470-646 lasso_tv, 878-953 admm) on the GPU."""
 import numpy as np
 import torch
 
@@ -32,8 +32,20 @@ def lasso_tv(x, b, tp, param):
     return (xd if isinstance(x, torch.Tensor) else xd.cpu().numpy()), err
 
 
+def lasso_l1(x, b, tp, param):
+    """admm.py:291-447: returns (x, err) with err[k] = ||x - u|| after outer step k."""
+    n1 = 200 if param.n_iter is None else int(param.n_iter)
+    xd = _dev(x).clone()
+    plan = tp.plan(xd.shape[-1])
+    xd, err = solvers.admm_l1(plan, xd, _dev(b), n1, n_inner=int(param.n_iter_pcg),
+                              lambd1=float(param.lambd1), lambd2=float(param.lambd2),
+                              mu=float(param.mu), pc=float(getattr(param, "pc", -1.0)))
+    xd = xd.reshape(x.shape) if isinstance(x, torch.Tensor) else xd.reshape(np.shape(x))
+    return (xd if isinstance(x, torch.Tensor) else xd.cpu().numpy()), err
+
+
 def admm(b, a, param):
-    """admm.py:878-953 for the radon projectors, regularisers 'cg' and 'lasso-tv';
+    """admm.py:878-953 for the radon projectors, regularisers 'cg', 'lasso' and 'lasso-tv';
     x starts from ones (admm.py:911)."""
     oper = PROJECTOR_OPER.get(param.projector, param.projector)
     n = param.npix_op
@@ -43,6 +55,8 @@ def admm(b, a, param):
     x = np.ones((b.shape[0], n, n), dtype=myfloat)
     if param.reg == "cg":
         return conj_grad(x, b, tp, param)
+    if param.reg == "lasso":
+        return lasso_l1(x, b, tp, param)
     if param.reg == "lasso-tv":
         return lasso_tv(x, b, tp, param)
     raise NotImplementedError("ADMM regulariser %r is outside the GPU path" % param.reg)

@@ -108,6 +108,10 @@ cg_hp_kernel(float* __restrict__ Hp, const float* __restrict__ p, const float* _
             h = __fadd_rn(__fadd_rn(h, __fmul_rn(lambd1, p[k])),
                           __fmul_rn(mu, gradT_at(g0, g0 + n2, n, i, j)));
             Hp[k] = h;
+        } else if (lambd1 != 0.0f) {
+            // cg.py:211: At(A p) + (lambd1 + mu) * p, the sum passed as lambd1
+            h = __fadd_rn(h, __fmul_rn(lambd1, p[k]));
+            Hp[k] = h;
         }
         s += (double)p[k] * (double)h;
     }
@@ -250,6 +254,51 @@ admm_tv_update_kernel(const float* __restrict__ x, float* __restrict__ u,
             al[kk] = __fadd_rn(al[kk], rr);
             s += (double)rr * (double)rr;
         }
+    }
+    block_partials3(s, 0.0, 0.0, partials + ((int64_t)z * kVecBlocks + blockIdx.x) * 3);
+}
+
+// Lasso-L1 inner-solve residual (cg.py:191): r = (A^T b + v) - (A^T A x + c x) with
+// c = lambd1 + mu; partial <r, r>.
+__global__ void __launch_bounds__(256)
+l1_residual_kernel(float* __restrict__ r, const float* __restrict__ atb,
+                   const float* __restrict__ atax, const float* __restrict__ x,
+                   const float* __restrict__ v, int64_t n2, float c,
+                   double* __restrict__ partials)
+{
+    const int z = blockIdx.y;
+    const int64_t off = (int64_t)z * n2;
+    double s = 0.0;
+    for (int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x; k < n2;
+         k += (int64_t)gridDim.x * 256) {
+        const float rv = __fsub_rn(__fadd_rn(atb[off + k], v[off + k]),
+                                   __fadd_rn(atax[off + k], __fmul_rn(c, x[off + k])));
+        r[off + k] = rv;
+        s += (double)rv * (double)rv;
+    }
+    block_partials3(s, 0.0, 0.0, partials + ((int64_t)z * kVecBlocks + blockIdx.x) * 3);
+}
+
+// Lasso-L1 multiplier step (admm.py:380-386): u = soft(x + alpha/mu, lambd2/mu)
+// (utils.hard_thres, utils.py:214-216), alpha += x - u; partial sum (x - u)^2.
+__global__ void __launch_bounds__(256)
+admm_l1_update_kernel(const float* __restrict__ x, float* __restrict__ u,
+                      float* __restrict__ al, int64_t n2, float mu, float lambd2,
+                      double* __restrict__ partials)
+{
+    const int z = blockIdx.y;
+    const int64_t off = (int64_t)z * n2;
+    const float t = __fdiv_rn(lambd2, mu);
+    double s = 0.0;
+    for (int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x; k < n2;
+         k += (int64_t)gridDim.x * 256) {
+        const float xv = x[off + k];
+        const float y = __fadd_rn(xv, __fdiv_rn(al[off + k], mu));
+        const float un = __fsub_rn(fmaxf(0.0f, __fsub_rn(y, t)), fmaxf(0.0f, __fsub_rn(-y, t)));
+        u[off + k] = un;
+        const float rr = __fsub_rn(xv, un);
+        al[off + k] = __fadd_rn(al[off + k], rr);
+        s += (double)rr * (double)rr;
     }
     block_partials3(s, 0.0, 0.0, partials + ((int64_t)z * kVecBlocks + blockIdx.x) * 3);
 }
@@ -429,6 +478,36 @@ int itomo_tv_rhs(float* v, const float* u, const float* alpha, int64_t n, float 
     if (!v || !u || !alpha || n < 1) return set_error(ITOMO_EINVAL, "tv_rhs: bad argument");
     tv_rhs_kernel<<<grid1(n), 256, 0, (hipStream_t)stream>>>(v, u, alpha, n, mu);
     return check_hip(hipGetLastError(), "tv_rhs launch");
+}
+
+int itomo_l1_residual(float* r, const float* atb, const float* atax, const float* x,
+                      const float* v, int npix, int nslices, float c, double* scal,
+                      double* work, void* stream)
+{
+    int rc = vec_check(npix, nslices, "l1_residual");
+    if (rc) return rc;
+    if (!r || !atb || !atax || !x || !v || !scal || !work)
+        return set_error(ITOMO_EINVAL, "l1_residual: null array");
+    hipStream_t st = (hipStream_t)stream;
+    dim3 grid(kVecBlocks, nslices);
+    l1_residual_kernel<<<grid, 256, 0, st>>>(r, atb, atax, x, v, (int64_t)npix * npix, c, work);
+    if ((rc = check_hip(hipGetLastError(), "l1_residual launch"))) return rc;
+    return finalize_into(work, nslices, scal, 0, st);
+}
+
+int itomo_admm_l1_update(const float* x, float* u, float* alpha, int npix, int nslices,
+                         float mu, float lambd2, double* scal, double* work, void* stream)
+{
+    int rc = vec_check(npix, nslices, "admm_l1_update");
+    if (rc) return rc;
+    if (!x || !u || !alpha || !scal || !work)
+        return set_error(ITOMO_EINVAL, "admm_l1_update: null array");
+    hipStream_t st = (hipStream_t)stream;
+    dim3 grid(kVecBlocks, nslices);
+    admm_l1_update_kernel<<<grid, 256, 0, st>>>(x, u, alpha, (int64_t)npix * npix, mu, lambd2,
+                                                work);
+    if ((rc = check_hip(hipGetLastError(), "admm_l1_update launch"))) return rc;
+    return finalize_into(work, nslices, scal, 3, st);
 }
 
 int itomo_admm_tv_update(const float* x, float* u, float* alpha, int npix, int nslices,

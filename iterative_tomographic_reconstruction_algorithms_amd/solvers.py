@@ -283,3 +283,39 @@ def admm_tv(plan, x, b, n_iter, n_inner=4, lambd1=1.0, lambd2=1.0, mu=1.0, pc=-1
                 "admm_tv_update")
         errs[it] = vw.scal.view(nz, 4)[:, 3].sum().sqrt()
     return x, errs.cpu().numpy()
+
+
+def admm_l1(plan, x, b, n_iter, n_inner=4, lambd1=1.0, lambd2=1.0, mu=1.0, pc=-1.0):
+    """admm.py:291-447 (lasso_l1): per outer step an inner CG of n_inner iterations on
+    (A^T A + (lambd1 + mu) I) x = A^T b + mu u - alpha (cg.py:189-245), then
+    u = soft(x + alpha/mu, lambd2/mu), alpha += x - u.  Returns (x, err) with
+    err[k] = ||x - u|| over all slices (admm.py:390)."""
+    x = _as3(x)
+    b = _as3(b)
+    nz, n = x.shape[0], x.shape[-1]
+    u = torch.zeros_like(x)
+    al = torch.zeros_like(x)
+    v = torch.empty_like(x)
+    vw = VecWork(nz)
+    lib = L.lib()
+    st = stream_handle()
+    c = float(np.float32(lambd1) + np.float32(mu))        # float32 sum, as in cg.py:191
+    atb = plan.back(b)
+    work = torch.empty_like(b)
+    atax = torch.empty_like(x)
+    r = torch.empty_like(x)
+    errs = torch.zeros(n_iter, dtype=torch.float64, device=x.device)
+    for it in range(n_iter):
+        L.check(lib.itomo_tv_rhs(_p(v), _p(u), _p(al), v.numel(), float(mu), st), "tv_rhs")
+        plan.forward(x, out=work)
+        plan.back(work, out=atax)
+        L.check(lib.itomo_l1_residual(_p(r), _p(atb), _p(atax), _p(x), _p(v), n, nz, c,
+                                      _p(vw.scal), _p(vw.work), st), "l1_residual")
+        p = r.clone()
+        _cg_loop(plan, x, r, p, n_inner, vw, lambd1=c, with_eps=False)  # cg.py:236
+        if pc >= 0.0:
+            x[x < 0] *= pc                                   # admm.py:364-374
+        L.check(lib.itomo_admm_l1_update(_p(x), _p(u), _p(al), n, nz, float(mu), float(lambd2),
+                                         _p(vw.scal), _p(vw.work), st), "admm_l1_update")
+        errs[it] = vw.scal.view(nz, 4)[:, 3].sum().sqrt()
+    return x, errs.cpu().numpy()

@@ -118,16 +118,20 @@ def gradient_adjoint(g):
 
 
 def cg(x0, b, A, At, n_iter, dtype=np.float32, lambd1=0.0, mu=0.0, v=None, tv=False,
-       with_eps=True):
-    """cg.py:83-160 (tv=False) and cg.py:256-306 (tv=True): CG on
-    (At A + lambd1 I + mu Gt G) x = At b (+ Gt v), vectors in `dtype`, inner products and
+       with_eps=True, l1=False):
+    """cg.py:83-160 (plain), cg.py:256-306 (tv=True) and cg.py:189-245 (l1=True): CG on
+    (At A + lambd1 I + mu Gt G) x = At b (+ Gt v), or (At A + (lambd1 + mu) I) x = At b + v, vectors in `dtype`, inner products and
     step sizes in float64 (the GPU path's FP64 scalars; numpy-1 rounds alpha and beta to
     the vector dtype before scaling).  Returns (x, rr) with rr[k] = <r_{k+1}, r_{k+1}>."""
     dt = dtype
     x = x0.astype(dt).copy()
 
+    c = dt(dt(lambd1) + dt(mu))        # cg.py:191,211: (lambd1 + mu), float32 params
+
     def H(p):
         h = At(A(p).astype(dt)).astype(dt)
+        if l1:
+            h = (h + c * p).astype(dt)
         if tv:
             gp = [g.astype(dt) for g in gradient(p)]
             h = (h + dt(lambd1) * p).astype(dt)
@@ -139,6 +143,10 @@ def cg(x0, b, A, At, n_iter, dtype=np.float32, lambd1=0.0, mu=0.0, v=None, tv=Fa
         lhs = (At(b).astype(dt) + gradient_adjoint(v).astype(dt)).astype(dt)
         rhs = ((At(A(x).astype(dt)).astype(dt) + dt(lambd1) * x).astype(dt)
                + gradient_adjoint(gx).astype(dt)).astype(dt)
+        r = (lhs - rhs).astype(dt)
+    elif l1:
+        lhs = (At(b).astype(dt) + v.astype(dt)).astype(dt)
+        rhs = (At(A(x).astype(dt)).astype(dt) + (c * x).astype(dt)).astype(dt)
         r = (lhs - rhs).astype(dt)
     else:
         r = (At(b).astype(dt) - At(A(x).astype(dt)).astype(dt)).astype(dt)
@@ -158,6 +166,27 @@ def cg(x0, b, A, At, n_iter, dtype=np.float32, lambd1=0.0, mu=0.0, v=None, tv=Fa
         p = (r + (bta * p).astype(dt)).astype(dt)
         rr = rn
     return x, np.array(out)
+
+
+def admm_l1(x0, b, A, At, n_iter, n_inner=4, lambd1=1.0, lambd2=1.0, mu=1.0,
+            dtype=np.float32):
+    """admm.py:291-447 for one slice: inner CG (cg.py:189-245, no eps in beta) then
+    u = soft(x + alpha/mu, lambd2/mu), alpha += x - u.  Returns (x, err) with
+    err[k] = ||x - u|| after outer step k."""
+    dt = dtype
+    x = x0.astype(dt).copy()
+    u = np.zeros_like(x)
+    al = np.zeros_like(x)
+    err = []
+    for _ in range(n_iter):
+        v = (dt(mu) * u - al).astype(dt)
+        x, _ = cg(x, b, A, At, n_inner, dtype=dt, lambd1=lambd1, mu=mu, v=v, l1=True,
+                  with_eps=False)
+        u = soft((x + (al / dt(mu)).astype(dt)).astype(dt), dt(lambd2) / dt(mu)).astype(dt)
+        r = (x - u).astype(dt)
+        al = (al + r).astype(dt)
+        err.append(np.sqrt(np.sum(r.astype(np.float64) ** 2)))
+    return x, np.array(err)
 
 
 def soft(y, t):

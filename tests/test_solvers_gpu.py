@@ -96,6 +96,25 @@ def test_admm_tv_matches_restatement():
     assert rel(x[0].cpu().numpy(), x64) <= 1e-3
 
 
+def test_admm_l1_matches_restatement():
+    n, v, it = 32, 30, 3
+    plan, A, At, A64, At64 = geometry(n, v)
+    b = A(ellipses(n, 6, seed=5))
+    x, err = G.admm_l1(plan, dev(np.ones((n, n))), dev(b), it, n_inner=4, lambd1=0.5,
+                       lambd2=0.2, mu=1.0)
+    x_ref, err_ref = S.admm_l1(np.ones((n, n), np.float32), b, A, At, it, 4, 0.5, 0.2, 1.0)
+    assert rel(x[0].cpu().numpy(), x_ref) <= 1e-4
+    np.testing.assert_allclose(err, err_ref, rtol=1e-3)
+    x64, _ = S.admm_l1(np.ones((n, n)), b.astype(np.float64), A64, At64, it, 4, 0.5, 0.2, 1.0,
+                       dtype=np.float64)
+    assert rel(x[0].cpu().numpy(), x64) <= 1e-3
+    # the multiplier step leaves u sparse: soft(x + alpha/mu, lambd2/mu)
+    p = param(n_iter=it, reg="lasso", lambd1=0.5, lambd2=0.2, mu=1.0, npix_op=n,
+              projector="dist-driv")
+    xa, info = admm_mod.admm(b[None], angles_deg(v), p)
+    assert rel(np.asarray(xa)[0], x_ref) <= 1e-4
+
+
 def test_dropin_solvers_numpy_interface():
     n, v = 48, 40
     deg = angles_deg(v)

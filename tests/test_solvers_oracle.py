@@ -155,3 +155,16 @@ def test_ramp_taps_are_twice_ram_lak():
     rl = np.where(n == 0, 0.25, np.where(n % 2 == 1, -1.0 / (np.pi * np.maximum(n, 1)) ** 2, 0.0))
     assert np.max(np.abs(h - 2 * rl)) < 2e-3
     assert abs(h[0] - 0.5) < 2e-3
+
+
+def test_admm_l1_reaches_the_l1_regularised_minimiser():
+    # with A = I (diagonal), the ADMM-L1 fixed point is the elastic-net solution
+    # x = soft(b, lambd2) / (1 + lambd1) componentwise
+    rng = np.random.default_rng(6)
+    b = rng.standard_normal((5, 5))
+    A = lambda x: x
+    At = lambda y: y
+    x, err = S.admm_l1(np.zeros((5, 5)), b, A, At, 400, n_inner=6, lambd1=0.5, lambd2=0.3,
+                       mu=1.0, dtype=np.float64)
+    np.testing.assert_allclose(x, S.soft(b, 0.3) / 1.5, atol=1e-6)
+    assert err[-1] < 1e-6
