@@ -378,6 +378,57 @@ __device__ __forceinline__ float first_pixel(float Dlo, float K, int nh, int npi
 
 constexpr int kFwdLines = 4;   // lines whose gathers are in flight together
 
+// Gather-form forward of one bin over lines [i_lo, i_hi] for views with at most NC
+// candidate pixels per bin and line, GL lines per group (see dd_fwd_block).
+template <int ARITH, bool ONES, int NC, int GL>
+__device__ __forceinline__ float fwd_grouped(const __amdgpu_buffer_rsrc_t rs, const DdView& g,
+                                             const float* __restrict__ ls, int i_lo, int i_hi,
+                                             int npix, int nh, float Dlo, float Dhi,
+                                             bool last_bin, float acc)
+{
+    const float xNm1 = (float)(npix - 1 - nh);
+    float u = line_coord(i_lo, nh);
+    int lbase = i_lo * npix;
+    int i = i_lo;
+    for (; i + GL - 1 <= i_hi; i += GL) {
+        float K[GL], xq[GL], v[GL][NC];
+#pragma unroll
+        for (int l = 0; l < GL; ++l) {
+            K[l] = key_K<ARITH>(g, u);
+            u = __fadd_rn(u, 1.0f);
+            xq[l] = first_pixel(Dlo, K[l], nh, npix);
+            const int q = (int)xq[l] + nh;
+#pragma unroll
+            for (int k = 0; k < NC; ++k) v[l][k] = ONES ? 1.0f : load_px(rs, lbase + q + k);
+            lbase += npix;
+        }
+#pragma unroll
+        for (int l = 0; l < GL; ++l) {
+            float Dhc = Dhi;
+            if (last_bin || Dhi > __fsub_rn(xNm1, K[l])) Dhc = fminf(Dhi, ls[i + l]);
+            float xk = xq[l], Plo = __fsub_rn(xq[l], K[l]);
+#pragma unroll
+            for (int k = 0; k < NC; ++k)
+                acc = cand_step<ARITH>(k == 0, xk, Plo, K[l], Dlo, Dhc, v[l][k], acc);
+        }
+    }
+    for (; i <= i_hi; ++i, u = __fadd_rn(u, 1.0f), lbase += npix) {
+        const float K = key_K<ARITH>(g, u);
+        const float x = first_pixel(Dlo, K, nh, npix);
+        const int q = (int)x + nh;
+        float Dhc = Dhi;
+        if (last_bin || Dhi > __fsub_rn(xNm1, K)) Dhc = fminf(Dhi, ls[i]);
+        float v[NC];
+#pragma unroll
+        for (int k = 0; k < NC; ++k) v[k] = ONES ? 1.0f : load_px(rs, lbase + q + k);
+        float xk = x, Plo = __fsub_rn(x, K);
+#pragma unroll
+        for (int k = 0; k < NC; ++k)
+            acc = cand_step<ARITH>(k == 0, xk, Plo, K, Dlo, Dhc, v[k], acc);
+    }
+    return acc;
+}
+
 // image[idx .. idx+3] of one slice through the buffer descriptor (out of range: 0).
 __device__ __forceinline__ float4 load_px4(__amdgpu_buffer_rsrc_t rs, int idx)
 {
@@ -540,49 +591,22 @@ __device__ __forceinline__ void dd_fwd_block(const DdView* __restrict__ geom,
             const float xNm1 = (float)(npix - 1 - nh);
             float u = line_coord(i_lo, nh);
             int lbase = i_lo * npix;
-            if (nc <= 3) {
-                // Lines in groups of kFwdLines: all their loads are issued before the first
-                // is consumed, so a wave keeps several gathers in flight; the lines are
-                // then accumulated in increasing order as before.
-                int i = i_lo;
-                for (; i + kFwdLines - 1 <= i_hi; i += kFwdLines) {
-                    float K[kFwdLines], xq[kFwdLines], v[kFwdLines][3];
-#pragma unroll
-                    for (int l = 0; l < kFwdLines; ++l) {
-                        K[l] = key_K<ARITH>(g, u);
-                        u = __fadd_rn(u, 1.0f);
-                        xq[l] = first_pixel(Dlo, K[l], nh, npix);
-                        const int q = (int)xq[l] + nh;
-#pragma unroll
-                        for (int k = 0; k < 3; ++k)
-                            v[l][k] = ONES ? 1.0f : load_px(rs, lbase + q + k);
-                        lbase += npix;
-                    }
-#pragma unroll
-                    for (int l = 0; l < kFwdLines; ++l) {
-                        float Dhc = Dhi;
-                        if (last_bin || Dhi > __fsub_rn(xNm1, K[l])) Dhc = fminf(Dhi, ls[i + l]);
-                        float xk = xq[l], Plo = __fsub_rn(xq[l], K[l]);
-#pragma unroll
-                        for (int k = 0; k < 3; ++k)
-                            acc = cand_step<ARITH>(k == 0, xk, Plo, K[l], Dlo, Dhc, v[l][k], acc);
-                    }
-                }
-                for (; i <= i_hi; ++i, u = __fadd_rn(u, 1.0f), lbase += npix) {
-                    const float K = key_K<ARITH>(g, u);
-                    const float x = first_pixel(Dlo, K, nh, npix);
-                    const int q = (int)x + nh;
-                    float Dhc = Dhi;
-                    if (last_bin || Dhi > __fsub_rn(xNm1, K)) Dhc = fminf(Dhi, ls[i]);
-                    float v[3];
-#pragma unroll
-                    for (int k = 0; k < 3; ++k) v[k] = ONES ? 1.0f : load_px(rs, lbase + q + k);
-                    float xk = x, Plo = __fsub_rn(x, K);
-#pragma unroll
-                    for (int k = 0; k < 3; ++k)
-                        acc = cand_step<ARITH>(k == 0, xk, Plo, K, Dlo, Dhc, v[k], acc);
-                }
-            } else {
+            // Up to 8 candidates: lines in groups, all their loads issued before the first
+            // is consumed, so a wave keeps several gathers in flight; the lines are then
+            // accumulated in increasing order.  Candidates past the bin add nothing.
+            if (nc <= 3)
+                acc = fwd_grouped<ARITH, ONES, 3, kFwdLines>(rs, g, ls, i_lo, i_hi, npix, nh, Dlo,
+                                                           Dhi, last_bin, acc);
+            else if (nc == 4)
+                acc = fwd_grouped<ARITH, ONES, 4, 4>(rs, g, ls, i_lo, i_hi, npix, nh, Dlo, Dhi,
+                                                     last_bin, acc);
+            else if (nc <= 6)
+                acc = fwd_grouped<ARITH, ONES, 6, 3>(rs, g, ls, i_lo, i_hi, npix, nh, Dlo, Dhi,
+                                                     last_bin, acc);
+            else if (nc <= 8)
+                acc = fwd_grouped<ARITH, ONES, 8, 2>(rs, g, ls, i_lo, i_hi, npix, nh, Dlo, Dhi,
+                                                     last_bin, acc);
+            else {
                 for (int i = i_lo; i <= i_hi; ++i, u = __fadd_rn(u, 1.0f), lbase += npix) {
                     const float K = key_K<ARITH>(g, u);
                     const float x = first_pixel(Dlo, K, nh, npix);
