@@ -19,6 +19,9 @@
 // No atomics: every output element has exactly one writer.  The SIRT/MLEM update of
 // the element is applied in the same thread right after its sum is complete (see
 // epilogue.h), with FP64 partial sums reduced per block in a fixed order.
+#include <vector>
+
+
 #include "dd_device.h"
 
 namespace itomo {
@@ -34,16 +37,19 @@ dd_back_kernel(const DdView* __restrict__ geom, const float* __restrict__ lstar,
                                    blockIdx.y * gridDim.x + blockIdx.x);
 }
 
-template <int ARITH, int EPI, bool ONES, int B>
+// views: the view of each blockIdx.y (null: blockIdx.y itself).  FP64 partial slots are
+// numbered by view, so launches over disjoint view lists fill one slot table.
+template <int ARITH, int EPI, bool ONES, int B, int PATH>
 __global__ void __launch_bounds__(B)
 dd_fwd_kernel(const DdView* __restrict__ geom, const float* __restrict__ lstar, int nang,
               int npix, const float* __restrict__ image, const float* __restrict__ imageT,
               float* __restrict__ out, int64_t img_stride, int64_t sino_stride,
-              int accumulate, EpiArgs e)
+              int accumulate, EpiArgs e, const int* __restrict__ views)
 {
-    dd_fwd_block<ARITH, EPI, ONES, B>(geom, lstar, nang, npix, image, imageT, out, img_stride,
-                                   sino_stride, accumulate, e, blockIdx.x, blockIdx.y,
-                                   blockIdx.z, blockIdx.y * gridDim.x + blockIdx.x);
+    const int view = views ? views[blockIdx.y] : (int)blockIdx.y;
+    dd_fwd_block<ARITH, EPI, ONES, B, PATH>(geom, lstar, nang, npix, image, imageT, out,
+                                            img_stride, sino_stride, accumulate, e, blockIdx.x,
+                                            view, blockIdx.z, view * gridDim.x + blockIdx.x);
 }
 
 __global__ void __launch_bounds__(256)
@@ -68,14 +74,33 @@ line_end_kernel(const DdView* __restrict__ geom, int nang, int npix, float* __re
 
 namespace {
 
+template <int ARITH, int EPI, bool ONES, int B, int PATH>
+void launch_fwd_path(const Plan& p, const float* image, const float* imageT, float* out,
+                     int nslices, int accumulate, const EpiArgs& e, const int* views,
+                     int nviews, hipStream_t st)
+{
+    dim3 block(B), grid((p.npix + B - 1) / B, nviews, nslices);
+    dd_fwd_kernel<ARITH, EPI, ONES, B, PATH><<<grid, block, 0, st>>>(
+        (const DdView*)p.d_geom, p.d_lstar, p.nang, p.npix, image, imageT, out,
+        (int64_t)p.npix * p.npix, (int64_t)p.nang * p.npix, accumulate, e, views);
+}
+
+// Views with more than 8 candidates per bin and line go to a launch of their own (see
+// kPathMain); with unit input, one launch takes every view.
 template <int ARITH, int EPI, bool ONES, int B>
 void launch_fwd_b(const Plan& p, const float* image, const float* imageT, float* out,
                   int nslices, int accumulate, const EpiArgs& e, hipStream_t st)
 {
-    dim3 block(B), grid((p.npix + B - 1) / B, p.nang, nslices);
-    dd_fwd_kernel<ARITH, EPI, ONES, B><<<grid, block, 0, st>>>(
-        (const DdView*)p.d_geom, p.d_lstar, p.nang, p.npix, image, imageT, out,
-        (int64_t)p.npix * p.npix, (int64_t)p.nang * p.npix, accumulate, e);
+    if (ONES || !p.d_vlist || p.n_gather == 0) {
+        launch_fwd_path<ARITH, EPI, ONES, B, kPathAll>(p, image, imageT, out, nslices,
+                                                       accumulate, e, nullptr, p.nang, st);
+        return;
+    }
+    if (p.n_staged > 0)
+        launch_fwd_path<ARITH, EPI, ONES, B, kPathMain>(p, image, imageT, out, nslices,
+                                                        accumulate, e, p.d_vlist, p.n_staged, st);
+    launch_fwd_path<ARITH, EPI, ONES, B, kPathWide>(p, image, imageT, out, nslices, accumulate,
+                                                    e, p.d_vlist + p.n_staged, p.n_gather, st);
 }
 
 template <int ARITH, int EPI, bool ONES>
@@ -225,6 +250,26 @@ int dd_back(const Plan& p, const float* sino, float* image, int nslices, int64_t
 {
     EpiArgs e{};
     return dd_back_epi(p, EPI_STORE, 0, sino, image, nslices, accumulate, e, st, nullptr);
+}
+
+int dd_build_view_lists(Plan& p)
+{
+    const DdView* g = (const DdView*)p.h_geom;
+    std::vector<int> v;
+    v.reserve(p.nang);
+    auto wide = [&](int k) {
+        return (g[k].kind == DD_SWEEP_ROWS || g[k].kind == DD_SWEEP_COLS) && g[k].ncand > 8;
+    };
+    for (int k = 0; k < p.nang; ++k)
+        if (!wide(k)) v.push_back(k);
+    p.n_staged = (int)v.size();
+    for (int k = 0; k < p.nang; ++k)
+        if (wide(k)) v.push_back(k);
+    p.n_gather = p.nang - p.n_staged;
+    int rc = check_hip(hipMalloc(&p.d_vlist, (size_t)p.nang * sizeof(int)), "plan_create: hipMalloc");
+    if (rc) return rc;
+    return check_hip(hipMemcpy(p.d_vlist, v.data(), (size_t)p.nang * sizeof(int),
+                               hipMemcpyHostToDevice), "plan_create: hipMemcpy");
 }
 
 int dd_build_line_ends(Plan& p)

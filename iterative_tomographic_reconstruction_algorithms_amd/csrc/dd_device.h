@@ -485,22 +485,36 @@ __device__ __forceinline__ float fwd_staged(const DdView& g, const float* __rest
             sQ0[tid] = ((int)est + nh - 2) & ~3;
         }
         __syncthreads();
-        for (int k = tid; k < ncl * (W / 4); k += B) {
+        // every thread loads its FL*W/(4B) window pieces into registers first, then
+        // stores them: the loads of one chunk are all in flight together
+        constexpr int PER = FL * (W / 4) / B;
+        float4 v[PER];
+#pragma unroll
+        for (int r = 0; r < PER; ++r) {
+            const int k = tid + r * B;
             const int l = k / (W / 4), j4 = k - l * (W / 4);
-            const int q = sQ0[l] + 4 * j4;
-            // a window that starts left of the row reads zeros there (negative index ->
-            // out of range); past the row's end it reads the next row: never weighted
-            const int idx = (c0 + l) * npix + q;
-            float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-            if (q >= 0 && idx + 3 < npix * npix) {
-                v = load_px4(rs, idx);
-            } else if (q >= 0) {  // straddles the end of the slice: per-element checks
-                v.x = load_px(rs, idx);
-                v.y = load_px(rs, idx + 1);
-                v.z = load_px(rs, idx + 2);
-                v.w = load_px(rs, idx + 3);
+            v[r] = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (l < ncl) {
+                const int q = sQ0[l] + 4 * j4;
+                // a window that starts left of the row reads zeros there (negative index
+                // -> out of range); past the row's end it reads the next row: never
+                // weighted
+                const int idx = (c0 + l) * npix + q;
+                if (q >= 0 && idx + 3 < npix * npix) {
+                    v[r] = load_px4(rs, idx);
+                } else if (q >= 0) {  // straddles the end of the slice: per-element checks
+                    v[r].x = load_px(rs, idx);
+                    v[r].y = load_px(rs, idx + 1);
+                    v[r].z = load_px(rs, idx + 2);
+                    v[r].w = load_px(rs, idx + 3);
+                }
             }
-            *(float4*)&sImg[l][4 * j4] = v;
+        }
+#pragma unroll
+        for (int r = 0; r < PER; ++r) {
+            const int k = tid + r * B;
+            const int l = k / (W / 4), j4 = k - l * (W / 4);
+            if (l < ncl) *(float4*)&sImg[l][4 * j4] = v[r];
         }
         __syncthreads();
         for (int l = 0; l < ncl; ++l, u = __fadd_rn(u, 1.0f)) {
@@ -523,7 +537,13 @@ __device__ __forceinline__ float fwd_staged(const DdView& g, const float* __rest
 // sweeps (and row sums) read the transposed image so that the lanes of a wave, which
 // hold consecutive bins, read consecutive addresses.
 // Bins [bx*B, +B) of view `view`, slice z, one per thread; blk indexes the partials.
-template <int ARITH, int EPI, bool ONES, int B>
+// Which views a forward launch handles (block-uniform): all of them, the views with at
+// most 8 candidate pixels per bin and line (kPathMain), or the wider ones (kPathWide).
+// The wide views' deep load groups need many registers; compiled into a kernel of their
+// own they leave the main kernel's occupancy alone.
+constexpr int kPathAll = 0, kPathMain = 1, kPathWide = 2;
+
+template <int ARITH, int EPI, bool ONES, int B, int PATH = kPathAll>
 __device__ __forceinline__ void dd_fwd_block(const DdView* __restrict__ geom,
                                              const float* __restrict__ lstar, int nang,
                                              int npix, const float* __restrict__ image,
@@ -543,7 +563,7 @@ __device__ __forceinline__ void dd_fwd_block(const DdView* __restrict__ geom,
     double p0 = 0.0;
 
     const bool sweep = g.kind == DD_SWEEP_ROWS || g.kind == DD_SWEEP_COLS;
-    if (!ONES && sweep && g.ncand <= 3) {
+    if (PATH != kPathWide && !ONES && sweep && g.ncand <= 3) {
         // block-uniform: every thread joins the LDS-staged path; the epilogue inputs
         // are loaded first so that their latency hides behind the sweep
         const EpiFwdIn pre = active ? epi_fwd_load<EPI>(o, soff, z, e) : EpiFwdIn{0.0f, 0.0f};
@@ -591,21 +611,27 @@ __device__ __forceinline__ void dd_fwd_block(const DdView* __restrict__ geom,
             const float xNm1 = (float)(npix - 1 - nh);
             float u = line_coord(i_lo, nh);
             int lbase = i_lo * npix;
-            // Up to 8 candidates: lines in groups, all their loads issued before the first
+            // Up to 16 candidates: lines in groups, all their loads issued before the first
             // is consumed, so a wave keeps several gathers in flight; the lines are then
             // accumulated in increasing order.  Candidates past the bin add nothing.
-            if (nc <= 3)
+            if (PATH != kPathWide && nc <= 3)
                 acc = fwd_grouped<ARITH, ONES, 3, kFwdLines>(rs, g, ls, i_lo, i_hi, npix, nh, Dlo,
                                                            Dhi, last_bin, acc);
-            else if (nc == 4)
-                acc = fwd_grouped<ARITH, ONES, 4, 4>(rs, g, ls, i_lo, i_hi, npix, nh, Dlo, Dhi,
+            else if (PATH != kPathWide && nc == 4)
+                acc = fwd_grouped<ARITH, ONES, 4, 6>(rs, g, ls, i_lo, i_hi, npix, nh, Dlo, Dhi,
                                                      last_bin, acc);
-            else if (nc <= 6)
-                acc = fwd_grouped<ARITH, ONES, 6, 3>(rs, g, ls, i_lo, i_hi, npix, nh, Dlo, Dhi,
+            else if (PATH != kPathWide && nc <= 6)
+                acc = fwd_grouped<ARITH, ONES, 6, 4>(rs, g, ls, i_lo, i_hi, npix, nh, Dlo, Dhi,
                                                      last_bin, acc);
-            else if (nc <= 8)
-                acc = fwd_grouped<ARITH, ONES, 8, 2>(rs, g, ls, i_lo, i_hi, npix, nh, Dlo, Dhi,
+            else if (PATH != kPathWide && nc <= 8)
+                acc = fwd_grouped<ARITH, ONES, 8, 3>(rs, g, ls, i_lo, i_hi, npix, nh, Dlo, Dhi,
                                                      last_bin, acc);
+            else if (PATH != kPathMain && nc <= 12)
+                acc = fwd_grouped<ARITH, ONES, 12, 2>(rs, g, ls, i_lo, i_hi, npix, nh, Dlo, Dhi,
+                                                      last_bin, acc);
+            else if (PATH != kPathMain && nc <= 16)
+                acc = fwd_grouped<ARITH, ONES, 16, 2>(rs, g, ls, i_lo, i_hi, npix, nh, Dlo, Dhi,
+                                                      last_bin, acc);
             else {
                 for (int i = i_lo; i <= i_hi; ++i, u = __fadd_rn(u, 1.0f), lbase += npix) {
                     const float K = key_K<ARITH>(g, u);

@@ -89,6 +89,8 @@ struct Plan {
     float* d_lstar;  // distance-driven: [view][line] start of the dropped segment
     int* d_range;    // slant-stacking: [view][bin] stepped range [u0, u1) (int2), clipped
                      // to the image
+    int* d_vlist;    // distance-driven: views of the main forward launch, then the wide ones
+    int n_staged, n_gather;  // (more than 8 candidate pixels per bin and line)
     int fwd_var;     // distance-driven launch shapes (kFwdBins / kBackTiles index)
     int back_var;
     int method;      // pixel-driven: 0 nearest bin, 1 linear split (radon_pd.c:57-85)
@@ -173,6 +175,8 @@ int check_hip(hipError_t e, const char* what);
 // launchers (dd.hip / rd.hip / iter.hip)
 // Tabulates Plan::d_lstar (allocates it; synchronous).
 int dd_build_line_ends(Plan& p);
+// Splits the views by forward launch into Plan::d_vlist (allocates it; synchronous).
+int dd_build_view_lists(Plan& p);
 int dd_forward(const Plan& p, const float* image, float* sino, int nslices,
                int64_t img_stride, int64_t sino_stride, int accumulate, hipStream_t st);
 int dd_back(const Plan& p, const float* sino, float* image, int nslices,

@@ -332,6 +332,7 @@ static int plan_create(itomo_plan** out, int projector, int npix, const float* a
         st = check_hip(hipMemcpy(p->d_geom, p->h_geom, bytes, hipMemcpyHostToDevice),
                        "plan_create: hipMemcpy");
     if (st == ITOMO_OK && projector == ITOMO_PROJ_DD) st = dd_build_line_ends(*p);
+    if (st == ITOMO_OK && projector == ITOMO_PROJ_DD) st = dd_build_view_lists(*p);
     if (st == ITOMO_OK && projector == ITOMO_PROJ_SS) {
         const size_t n = (size_t)nang * npix * 2;
         std::vector<int> tab;
@@ -352,6 +353,7 @@ static int plan_create(itomo_plan** out, int projector, int npix, const float* a
                            "plan_create: hipMemcpy");
     }
     if (st != ITOMO_OK) {
+        if (p->d_vlist) (void)hipFree(p->d_vlist);
         if (p->d_range) (void)hipFree(p->d_range);
         if (p->d_lstar) (void)hipFree(p->d_lstar);
         if (p->d_geom) (void)hipFree(p->d_geom);
@@ -392,6 +394,10 @@ int itomo_plan_destroy(itomo_plan* p)
     }
     if (p->d_range) {
         int st2 = check_hip(hipFree(p->d_range), "plan_destroy: hipFree");
+        if (!st) st = st2;
+    }
+    if (p->d_vlist) {
+        int st2 = check_hip(hipFree(p->d_vlist), "plan_destroy: hipFree");
         if (!st) st = st2;
     }
     if (p->scratch) {
