@@ -273,8 +273,11 @@ def main():
     gups = 2.0 * N * N * V * K / secs / 1e9
     resid_rel = float(stats[0].sqrt()) / bnorm
 
-    fwd_name = "dd_fwd_kernel<FAST,STORE,%d> + fwd_epi_kernel<SIRT_FWD>" % (
-        [256, 128, 64][shapes["fwd"]])
+    # the forward half: transpose for the column sweeps, the projection in two launches
+    # (views with at most 8 candidate pixels per bin and line, then the wider ones), the
+    # SIRT update pass
+    fwd_name = ("transpose + dd_fwd_kernel<FAST,STORE,%d> (main, wide) + fwd_epi_kernel<SIRT_FWD>"
+                % ([256, 128, 64][shapes["fwd"]]))
     back_name = "dd_back_kernel<FAST,SIRT_BACK,%d,%d>" % tuple(
         [(64, 4), (32, 2), (32, 4)][shapes["back"]])
     if back_ms >= fwd_ms:

@@ -457,12 +457,11 @@ __device__ __forceinline__ float fwd_staged(const DdView& g, const float* __rest
     constexpr int FL = 8;  // 8 x 2B floats = 64B bytes per block: 1 KB per wave, full occupancy
     __shared__ __attribute__((aligned(16))) float sImg[FL][W];
     __shared__ int sQ0[FL];
+    __shared__ float sL[FL];   // L* of the chunk's lines
     const int tid = threadIdx.x;
     const int dk = min(d, npix - 1);
     const float Dlo = key_D<ARITH>(g, dk - nh);
     const float Dhi = key_D<ARITH>(g, dk + 1 - nh);
-    const bool last_bin = dk == npix - 1;
-    const float xNm1 = (float)(npix - 1 - nh);
     const int d0 = bx * B, d1 = min(d0 + B - 1, npix - 1);
     const float D0lo = key_D<ARITH>(g, d0 - nh);
     const float D1hi = key_D<ARITH>(g, d1 + 1 - nh);
@@ -483,6 +482,7 @@ __device__ __forceinline__ float fwd_staged(const DdView& g, const float* __rest
             const float est = fminf(fmaxf(floorf(__fadd_rn(D0lo, K)), (float)(-nh - W)),
                                     (float)(npix - nh + W));
             sQ0[tid] = ((int)est + nh - 2) & ~3;
+            sL[tid] = ls[c0 + tid];
         }
         __syncthreads();
         // every thread loads its FL*W/(4B) window pieces into registers first, then
@@ -517,16 +517,21 @@ __device__ __forceinline__ float fwd_staged(const DdView& g, const float* __rest
             if (l < ncl) *(float4*)&sImg[l][4 * j4] = v[r];
         }
         __syncthreads();
-        for (int l = 0; l < ncl; ++l, u = __fadd_rn(u, 1.0f)) {
-            const float K = key_K<ARITH>(g, u);
-            const float x = first_pixel(Dlo, K, nh, npix);
-            const int off = min(max((int)x + nh - sQ0[l], 0), W - 3);
-            float Dhc = Dhi;
-            if (last_bin || Dhi > __fsub_rn(xNm1, K)) Dhc = fminf(Dhi, ls[c0 + l]);
-            float xk = x, Plo = __fsub_rn(x, K);
 #pragma unroll
-            for (int k = 0; k < 3; ++k)
-                acc = cand_step<ARITH>(k == 0, xk, Plo, K, Dlo, Dhc, sImg[l][off + k], acc);
+        for (int l = 0; l < FL; ++l) {
+            if (l < ncl) {
+                const float K = key_K<ARITH>(g, u);
+                const float x = first_pixel(Dlo, K, nh, npix);
+                const int off = min(max((int)x + nh - sQ0[l], 0), W - 3);
+                // clipping every weight's right end to L* is the exact rule; it changes
+                // only the weights that end at the line's dropped segment
+                const float Dhc = fminf(Dhi, sL[l]);
+                float xk = x, Plo = __fsub_rn(x, K);
+#pragma unroll
+                for (int k = 0; k < 3; ++k)
+                    acc = cand_step<ARITH>(k == 0, xk, Plo, K, Dlo, Dhc, sImg[l][off + k], acc);
+            }
+            u = __fadd_rn(u, 1.0f);
         }
     }
     return acc;
