@@ -135,11 +135,16 @@ __device__ __forceinline__ void staged_range(const DdView& g, int npix, int nh, 
 // this view (zero outside [0, npix)); Ls is the line's L*.
 //
 // Bins are at least as wide as pixels (1/|sin| or 1/|cos| >= 1), so a pixel crosses at
-// most one bin boundary, except when the two widths agree to rounding: the crossing is
-// evaluated with selects, and a wave-uniform loop picks up the rare further ones.  Bins
-// are visited in increasing order either way.  Past the last bin the weights vanish by
-// the L* clip (Ls <= D[N]), so the walk needs no bound there.
-template <int ARITH, int P, int STRIDE>
+// most one bin boundary, except when the two widths agree to rounding (views flagged
+// `multi`, the MULTI instance): there a loop picks up the further crossings.  Bins are
+// visited in increasing order either way.  Past the last bin the weights vanish by the
+// L* clip (Ls <= D[N]), so the walk needs no bound there.
+//
+// The walk keeps the bin that holds the pixel's left end lo (D_lo <= lo < D_hi), so
+// every overlap starts at lo or at a crossed key; the one exception, a left end below
+// bin -1, reads a staged zero.  A weight max(R - L, 0) of zero leaves the sum unchanged,
+// so no overlap needs a test.
+template <int ARITH, int P, int STRIDE, bool MULTI>
 __device__ __forceinline__ void walk_line(const DdView& g, int npix, int nh, float K, float Ls,
                                           int p0, const float* S, float* a)
 {
@@ -156,35 +161,37 @@ __device__ __forceinline__ void walk_line(const DdView& g, int npix, int nh, flo
         fd = __fsub_rn(__fadd_rn(fd, up), dn);
         fd = fminf(fmaxf(fd, (float)(-nh - 1)), (float)(npix - nh));
     }
-    int d = (int)fd + nh;
-    float Dlo = key_Df<ARITH>(g, fd);
-    float Dhi = key_Df<ARITH>(g, __fadd_rn(fd, 1.0f));
+    // fr = (index - nh) of the current bin's right key Dhi; the bin is (int)fr + nh - 1
+    // (fr is an integer-valued float, the conversion exact)
+    float fr = __fadd_rn(fd, 1.0f);
+    float Dhi = key_Df<ARITH>(g, fr);
+    const int dbase = nh - 1;
 #define ITOMO_PIX(acc)                                                                  \
     {                                                                                   \
         x = __fadd_rn(x, 1.0f);                                                         \
         const float hi = __fsub_rn(x, K);                                               \
         const float hc = fminf(hi, Ls);                                                 \
+        const int d = (int)fr + dbase;                                                  \
         const float s0 = S[d], s1 = S[d + 1];                                           \
-        const float R = fminf(hc, Dhi), L = fmaxf(lo, Dlo);                             \
-        const float n0 = accum<ARITH>(__fsub_rn(R, L), s0, acc);                        \
-        acc = (R > L) ? n0 : acc;                                                       \
-        const float fd2 = __fadd_rn(fd, 1.0f);                                          \
-        const float Dhi2 = key_Df<ARITH>(g, __fadd_rn(fd2, 1.0f));                      \
+        acc = accum<ARITH>(fmaxf(__fsub_rn(fminf(hc, Dhi), lo), 0.0f), s0, acc);        \
+        const float fr2 = __fadd_rn(fr, 1.0f);                                          \
+        const float Dhi2 = key_Df<ARITH>(g, fr2);                                       \
+        /* one crossing at most: the span ends in (Dhi, Dhi2] when it crosses */        \
+        const float R2 = MULTI ? fminf(hc, Dhi2) : hc;                                  \
+        acc = accum<ARITH>(fmaxf(__fsub_rn(R2, Dhi), 0.0f), s1, acc);                   \
         const bool cross = Dhi < hi;                                                    \
-        const float R2 = fminf(hc, Dhi2);                                               \
-        const float n1 = accum<ARITH>(__fsub_rn(R2, Dhi), s1, acc);                     \
-        acc = (cross && R2 > Dhi) ? n1 : acc;                                           \
-        fd = cross ? fd2 : fd;                                                          \
-        d = cross ? d + 1 : d;                                                          \
-        Dlo = cross ? Dhi : Dlo;                                                        \
+        fr = cross ? fr2 : fr;                                                          \
         Dhi = cross ? Dhi2 : Dhi;                                                       \
-        while (d < npix && Dhi < hi) {                                                  \
-            ++d;                                                                        \
-            fd = __fadd_rn(fd, 1.0f);                                                   \
-            Dlo = Dhi;                                                                  \
-            Dhi = key_Df<ARITH>(g, __fadd_rn(fd, 1.0f));                                \
-            const float R3 = fminf(hc, Dhi);                                            \
-            if (R3 > Dlo) acc = accum<ARITH>(__fsub_rn(R3, Dlo), S[d], acc);            \
+        if (MULTI) {                                                                    \
+            int dm = d + 1;                                                             \
+            while (cross && dm < npix && Dhi < hi) {                                    \
+                ++dm;                                                                   \
+                const float Dlo = Dhi;                                                  \
+                fr = __fadd_rn(fr, 1.0f);                                               \
+                Dhi = key_Df<ARITH>(g, fr);                                             \
+                acc = accum<ARITH>(fmaxf(__fsub_rn(fminf(hc, Dhi), Dlo), 0.0f), S[dm],  \
+                                   acc);                                                \
+            }                                                                           \
         }                                                                               \
         lo = hi;                                                                        \
     }
@@ -276,15 +283,23 @@ __device__ __forceinline__ void dd_back_tile(const DdView* __restrict__ geom,
 #pragma unroll
                 for (int i = 0; i < P; ++i) {
                     const float K = key_K<ARITH>(g, line_coord(rr + i, nh));
-                    walk_line<ARITH, P, 1>(g, npix, nh, K, sLs[v][P * ty + i], cc, S,
-                                           &acc[i][0]);
+                    if (g.multi)
+                        walk_line<ARITH, P, 1, true>(g, npix, nh, K, sLs[v][P * ty + i], cc,
+                                                     S, &acc[i][0]);
+                    else
+                        walk_line<ARITH, P, 1, false>(g, npix, nh, K, sLs[v][P * ty + i], cc,
+                                                      S, &acc[i][0]);
                 }
             } else if (g.kind == DD_SWEEP_COLS) {
 #pragma unroll
                 for (int j = 0; j < P; ++j) {
                     const float K = key_K<ARITH>(g, line_coord(cc + j, nh));
-                    walk_line<ARITH, P, P>(g, npix, nh, K, sLs[v][P * tx + j], rr, S,
-                                           &acc[0][j]);
+                    if (g.multi)
+                        walk_line<ARITH, P, P, true>(g, npix, nh, K, sLs[v][P * tx + j], rr,
+                                                     S, &acc[0][j]);
+                    else
+                        walk_line<ARITH, P, P, false>(g, npix, nh, K, sLs[v][P * tx + j], rr,
+                                                      S, &acc[0][j]);
                 }
             } else if (g.kind == DD_ROWSUM) {
 #pragma unroll
@@ -346,7 +361,9 @@ __device__ __forceinline__ float load_px(__amdgpu_buffer_rsrc_t rs, int idx)
 
 // One candidate pixel of a bin [Dlo, Dhi] on a line with offset K: the pixel spans
 // [Plo, x+1-K].  Only the first candidate can start left of Dlo.  Dhc is the bin's right
-// end clipped to the line's L*.  Advances x and Plo.
+// end clipped to the line's L*.  Advances x and Plo.  A candidate that misses the bin
+// gets the weight max(R - L, 0) = 0, which leaves the sum unchanged (every candidate
+// value is a finite image value or a zero), so no overlap test is needed.
 template <int ARITH>
 __device__ __forceinline__ float cand_step(bool first, float& x, float& Plo, float K,
                                            float Dlo, float Dhc, float val, float acc)
@@ -355,9 +372,8 @@ __device__ __forceinline__ float cand_step(bool first, float& x, float& Plo, flo
     const float Phi = __fsub_rn(x, K);
     const float R = fminf(Phi, Dhc);
     const float L = first ? fmaxf(Plo, Dlo) : Plo;
-    const float nacc = accum<ARITH>(__fsub_rn(R, L), val, acc);
     Plo = Phi;
-    return R > L ? nacc : acc;
+    return accum<ARITH>(fmaxf(__fsub_rn(R, L), 0.0f), val, acc);
 }
 
 // (index - nh) of the pixel whose span [P_q, P_q+1) holds Dlo on a line with offset K,
@@ -367,13 +383,14 @@ __device__ __forceinline__ float first_pixel(float Dlo, float K, int nh, int npi
     // the estimate needs no clamp of its own: far outside the line the correction may
     // be inexact, but the final clamp then lands on the end pixel either way
     float fe = floorf(__fadd_rn(Dlo, K));
+    const float fe1 = __fadd_rn(fe, 1.0f);
     const float Pe = __fsub_rn(fe, K);
-    const float Pe1 = __fsub_rn(__fadd_rn(fe, 1.0f), K);
-    // selects, not branches: at most one of the two corrections applies
-    const float up = (Dlo >= Pe1) ? 1.0f : 0.0f;
-    const float dn = (Dlo < Pe) ? 1.0f : 0.0f;
-    fe = __fsub_rn(__fadd_rn(fe, up), dn);
-    return fminf(fmaxf(fe, (float)(-nh)), (float)(npix - 1 - nh));
+    const float Pe1 = __fsub_rn(fe1, K);
+    // selects, not branches: at most one of the two corrections applies (Pe <= Pe1)
+    const float fm1 = __fsub_rn(fe, 1.0f);
+    fe = (Dlo >= Pe1) ? fe1 : fe;
+    fe = (Dlo < Pe) ? fm1 : fe;
+    return __builtin_amdgcn_fmed3f(fe, (float)(-nh), (float)(npix - 1 - nh));
 }
 
 constexpr int kFwdLines = 4;   // lines whose gathers are in flight together
@@ -457,6 +474,7 @@ __device__ __forceinline__ float fwd_staged(const DdView& g, const float* __rest
     constexpr int FL = 8;  // 8 x 2B floats = 64B bytes per block: 1 KB per wave, full occupancy
     __shared__ __attribute__((aligned(16))) float sImg[FL][W];
     __shared__ int sQ0[FL];
+    __shared__ int sQb[FL];    // 4 * (nh - sQ0): byte offset of pixel index - nh
     __shared__ float sL[FL];   // L* of the chunk's lines
     const int tid = threadIdx.x;
     const int dk = min(d, npix - 1);
@@ -481,7 +499,9 @@ __device__ __forceinline__ float fwd_staged(const DdView& g, const float* __rest
             const float K = key_K<ARITH>(g, line_coord(c0 + tid, nh));
             const float est = fminf(fmaxf(floorf(__fadd_rn(D0lo, K)), (float)(-nh - W)),
                                     (float)(npix - nh + W));
-            sQ0[tid] = ((int)est + nh - 2) & ~3;
+            const int q0 = ((int)est + nh - 2) & ~3;
+            sQ0[tid] = q0;
+            sQb[tid] = 4 * (nh - q0);
             sL[tid] = ls[c0 + tid];
         }
         __syncthreads();
@@ -522,14 +542,16 @@ __device__ __forceinline__ float fwd_staged(const DdView& g, const float* __rest
             if (l < ncl) {
                 const float K = key_K<ARITH>(g, u);
                 const float x = first_pixel(Dlo, K, nh, npix);
-                const int off = min(max((int)x + nh - sQ0[l], 0), W - 3);
+                // byte offset of the first candidate in the line's window
+                const int ob = min(max(((int)x << 2) + sQb[l], 0), 4 * (W - 3));
+                const float* wp = (const float*)((const char*)&sImg[l][0] + ob);
                 // clipping every weight's right end to L* is the exact rule; it changes
                 // only the weights that end at the line's dropped segment
                 const float Dhc = fminf(Dhi, sL[l]);
                 float xk = x, Plo = __fsub_rn(x, K);
 #pragma unroll
                 for (int k = 0; k < 3; ++k)
-                    acc = cand_step<ARITH>(k == 0, xk, Plo, K, Dlo, Dhc, sImg[l][off + k], acc);
+                    acc = cand_step<ARITH>(k == 0, xk, Plo, K, Dlo, Dhc, wp[k], acc);
             }
             u = __fadd_rn(u, 1.0f);
         }

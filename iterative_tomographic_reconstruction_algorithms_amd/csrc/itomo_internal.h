@@ -39,7 +39,7 @@ struct DdView {
     float est;   // b, used only to estimate a bin index (result is corrected exactly)
     float s, c;  // sin, cos of the view as the reference computes them
     int ncand;   // pixels that can overlap one bin on one line: floor(rinv + 0.25) + 2
-    float pad;
+    int multi;   // a pixel's span can hold two bin keys (bin width within rounding of 1)
 };
 
 struct RdView {

@@ -47,7 +47,18 @@ static void view_trig(float theta, int arith, float* s, float* c)
 }
 
 // Branch selection and constants of radon_dd.c:69-283 for one view.
-static int make_dd_view(float theta, int arith, int row, DdView* g)
+// Whether a pixel span on a sweep line can hold two bin keys.  A span is at most
+// 1 + ulp(M) long and two consecutive bin keys are at least r - ulp(M) apart, r = 1/|b|,
+// with M bounding every key of the view, so one key per span is certain when
+// r - 1 > 2 ulp(M); the flag keeps a factor 2 of margin.
+static int dd_multi_cross(const DdView& g, int npix)
+{
+    const double r = std::fmin((double)g.rinv, 1.0 / std::fabs((double)g.b));
+    const double m = (npix + 4.0) * (1.0 + std::fabs((double)g.t) + r);
+    return (r - 1.0) <= 4.0 * m * std::ldexp(1.0, -23) ? 1 : 0;
+}
+
+static int make_dd_view(float theta, int arith, int row, int npix, DdView* g)
 {
     float s, c;
     view_trig(theta, arith, &s, &c);
@@ -88,6 +99,7 @@ static int make_dd_view(float theta, int arith, int row, DdView* g)
     } else {
         g->kind = DD_NONE;
     }
+    if (g->kind == DD_SWEEP_ROWS || g->kind == DD_SWEEP_COLS) g->multi = dd_multi_cross(*g, npix);
     return ITOMO_OK;
 }
 
@@ -300,7 +312,7 @@ static int plan_create(itomo_plan** out, int projector, int npix, const float* a
         DdView* h = (DdView*)std::malloc(bytes);
         if (!h) { delete p; return set_error(ITOMO_ENOMEM, "plan_create: out of host memory"); }
         for (int v = 0; v < nang && st == ITOMO_OK; ++v)
-            st = make_dd_view(angles[v], arith, nang - 1 - v, &h[v]);
+            st = make_dd_view(angles[v], arith, nang - 1 - v, npix, &h[v]);
         p->h_geom = h;
     } else if (projector == ITOMO_PROJ_SS) {
         bytes = sizeof(SsView) * (size_t)nang;
