@@ -258,7 +258,8 @@ int dd_build_view_lists(Plan& p)
     std::vector<int> v;
     v.reserve(p.nang);
     auto wide = [&](int k) {
-        return (g[k].kind == DD_SWEEP_ROWS || g[k].kind == DD_SWEEP_COLS) && g[k].ncand > 8;
+        return (g[k].kind == DD_SWEEP_ROWS || g[k].kind == DD_SWEEP_COLS) &&
+               g[k].ncand > kFwdMainMaxCand;
     };
     for (int k = 0; k < p.nang; ++k)
         if (!wide(k)) v.push_back(k);

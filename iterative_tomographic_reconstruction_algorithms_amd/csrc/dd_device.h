@@ -359,6 +359,43 @@ __device__ __forceinline__ float load_px(__amdgpu_buffer_rsrc_t rs, int idx)
         (int)__builtin_amdgcn_raw_buffer_load_b32(rs, (int)((unsigned)idx * 4u), 0, 0));
 }
 
+// image[idx .. idx+3] of one slice through the buffer descriptor (out of range: 0).
+// idx needs no 16-byte alignment.
+__device__ __forceinline__ float4 load_px4(__amdgpu_buffer_rsrc_t rs, int idx)
+{
+    typedef int int4v __attribute__((ext_vector_type(4)));
+    const int4v v = __builtin_amdgcn_raw_buffer_load_b128(rs, (int)((unsigned)idx * 4u), 0, 0);
+    return make_float4(__int_as_float(v[0]), __int_as_float(v[1]), __int_as_float(v[2]),
+                       __int_as_float(v[3]));
+}
+
+// v[0 .. NC) = image[idx .. idx+NC) of one slice (total pixels; out of range: 0), idx >= 0,
+// with 16-byte loads: a bin's candidates on one line are consecutive pixels, so one load
+// instruction fetches four of them.  A run that reaches the end of the slice is read
+// element by element.
+template <int NC>
+__device__ __forceinline__ void load_run(__amdgpu_buffer_rsrc_t rs, int idx, int total,
+                                         float* v)
+{
+#pragma unroll
+    for (int k4 = 0; k4 < NC; k4 += 4) {
+        const int i = idx + k4;
+        float4 f;
+        if (i + 3 < total) {
+            f = load_px4(rs, i);
+        } else {
+            f.x = load_px(rs, i);
+            f.y = load_px(rs, i + 1);
+            f.z = load_px(rs, i + 2);
+            f.w = load_px(rs, i + 3);
+        }
+        v[k4] = f.x;
+        if (k4 + 1 < NC) v[k4 + 1] = f.y;
+        if (k4 + 2 < NC) v[k4 + 2] = f.z;
+        if (k4 + 3 < NC) v[k4 + 3] = f.w;
+    }
+}
+
 // One candidate pixel of a bin [Dlo, Dhi] on a line with offset K: the pixel spans
 // [Plo, x+1-K].  Only the first candidate can start left of Dlo.  Dhc is the bin's right
 // end clipped to the line's L*.  Advances x and Plo.  A candidate that misses the bin
@@ -395,6 +432,10 @@ __device__ __forceinline__ float first_pixel(float Dlo, float K, int nh, int npi
 
 constexpr int kFwdLines = 4;   // lines whose gathers are in flight together
 
+// Most candidate pixels per bin and line of a view in the main forward launch; wider
+// views run in a launch of their own and read their candidates with 16-byte loads.
+constexpr int kFwdMainMaxCand = 8;
+
 // Gather-form forward of one bin over lines [i_lo, i_hi] for views with at most NC
 // candidate pixels per bin and line, GL lines per group (see dd_fwd_block).
 template <int ARITH, bool ONES, int NC, int GL>
@@ -415,8 +456,12 @@ __device__ __forceinline__ float fwd_grouped(const __amdgpu_buffer_rsrc_t rs, co
             u = __fadd_rn(u, 1.0f);
             xq[l] = first_pixel(Dlo, K[l], nh, npix);
             const int q = (int)xq[l] + nh;
+            if (ONES || NC <= kFwdMainMaxCand) {
 #pragma unroll
-            for (int k = 0; k < NC; ++k) v[l][k] = ONES ? 1.0f : load_px(rs, lbase + q + k);
+                for (int k = 0; k < NC; ++k) v[l][k] = ONES ? 1.0f : load_px(rs, lbase + q + k);
+            } else {
+                load_run<NC>(rs, lbase + q, npix * npix, v[l]);
+            }
             lbase += npix;
         }
 #pragma unroll
@@ -436,23 +481,18 @@ __device__ __forceinline__ float fwd_grouped(const __amdgpu_buffer_rsrc_t rs, co
         float Dhc = Dhi;
         if (last_bin || Dhi > __fsub_rn(xNm1, K)) Dhc = fminf(Dhi, ls[i]);
         float v[NC];
+        if (ONES || NC <= kFwdMainMaxCand) {
 #pragma unroll
-        for (int k = 0; k < NC; ++k) v[k] = ONES ? 1.0f : load_px(rs, lbase + q + k);
+            for (int k = 0; k < NC; ++k) v[k] = ONES ? 1.0f : load_px(rs, lbase + q + k);
+        } else {
+            load_run<NC>(rs, lbase + q, npix * npix, v);
+        }
         float xk = x, Plo = __fsub_rn(x, K);
 #pragma unroll
         for (int k = 0; k < NC; ++k)
             acc = cand_step<ARITH>(k == 0, xk, Plo, K, Dlo, Dhc, v[k], acc);
     }
     return acc;
-}
-
-// image[idx .. idx+3] of one slice through the buffer descriptor (out of range: 0).
-__device__ __forceinline__ float4 load_px4(__amdgpu_buffer_rsrc_t rs, int idx)
-{
-    typedef int int4v __attribute__((ext_vector_type(4)));
-    const int4v v = __builtin_amdgcn_raw_buffer_load_b128(rs, (int)((unsigned)idx * 4u), 0, 0);
-    return make_float4(__int_as_float(v[0]), __int_as_float(v[1]), __int_as_float(v[2]),
-                       __int_as_float(v[3]));
 }
 
 // Forward projection of one sweep view whose bins are narrower than 1.75 pixels (at
@@ -570,6 +610,12 @@ __device__ __forceinline__ float fwd_staged(const DdView& g, const float* __rest
 // own they leave the main kernel's occupancy alone.
 constexpr int kPathAll = 0, kPathMain = 1, kPathWide = 2;
 
+// Whether the grouped gather with NC candidates is compiled into launch PATH.
+constexpr bool fwd_in_path(int path, int nc)
+{
+    return path == kPathAll || ((path == kPathMain) == (nc <= kFwdMainMaxCand));
+}
+
 template <int ARITH, int EPI, bool ONES, int B, int PATH = kPathAll>
 __device__ __forceinline__ void dd_fwd_block(const DdView* __restrict__ geom,
                                              const float* __restrict__ lstar, int nang,
@@ -641,22 +687,22 @@ __device__ __forceinline__ void dd_fwd_block(const DdView* __restrict__ geom,
             // Up to 16 candidates: lines in groups, all their loads issued before the first
             // is consumed, so a wave keeps several gathers in flight; the lines are then
             // accumulated in increasing order.  Candidates past the bin add nothing.
-            if (PATH != kPathWide && nc <= 3)
+            if (fwd_in_path(PATH, 3) && nc <= 3)
                 acc = fwd_grouped<ARITH, ONES, 3, kFwdLines>(rs, g, ls, i_lo, i_hi, npix, nh, Dlo,
                                                            Dhi, last_bin, acc);
-            else if (PATH != kPathWide && nc == 4)
+            else if (fwd_in_path(PATH, 4) && nc == 4)
                 acc = fwd_grouped<ARITH, ONES, 4, 6>(rs, g, ls, i_lo, i_hi, npix, nh, Dlo, Dhi,
                                                      last_bin, acc);
-            else if (PATH != kPathWide && nc <= 6)
+            else if (fwd_in_path(PATH, 6) && nc <= 6)
                 acc = fwd_grouped<ARITH, ONES, 6, 4>(rs, g, ls, i_lo, i_hi, npix, nh, Dlo, Dhi,
                                                      last_bin, acc);
-            else if (PATH != kPathWide && nc <= 8)
+            else if (fwd_in_path(PATH, 8) && nc <= 8)
                 acc = fwd_grouped<ARITH, ONES, 8, 3>(rs, g, ls, i_lo, i_hi, npix, nh, Dlo, Dhi,
                                                      last_bin, acc);
-            else if (PATH != kPathMain && nc <= 12)
+            else if (fwd_in_path(PATH, 12) && nc <= 12)
                 acc = fwd_grouped<ARITH, ONES, 12, 2>(rs, g, ls, i_lo, i_hi, npix, nh, Dlo, Dhi,
                                                       last_bin, acc);
-            else if (PATH != kPathMain && nc <= 16)
+            else if (fwd_in_path(PATH, 16) && nc <= 16)
                 acc = fwd_grouped<ARITH, ONES, 16, 2>(rs, g, ls, i_lo, i_hi, npix, nh, Dlo, Dhi,
                                                       last_bin, acc);
             else {
@@ -668,10 +714,8 @@ __device__ __forceinline__ void dd_fwd_block(const DdView* __restrict__ geom,
                     if (last_bin || Dhi > __fsub_rn(xNm1, K)) Dhc = fminf(Dhi, ls[i]);
                     float xk = x, Plo = __fsub_rn(x, K);
                     for (int k0 = 0; k0 < nc; k0 += 4) {
-                        float v[4];
-#pragma unroll
-                        for (int k = 0; k < 4; ++k)
-                            v[k] = ONES ? 1.0f : load_px(rs, lbase + q + k0 + k);
+                        float v[4] = {1.0f, 1.0f, 1.0f, 1.0f};
+                        if (!ONES) load_run<4>(rs, lbase + q + k0, npix * npix, v);
 #pragma unroll
                         for (int k = 0; k < 4; ++k)
                             acc = cand_step<ARITH>(k0 + k == 0, xk, Plo, K, Dlo, Dhc, v[k], acc);
