@@ -495,27 +495,37 @@ __device__ __forceinline__ float fwd_grouped(const __amdgpu_buffer_rsrc_t rs, co
     return acc;
 }
 
-// Forward projection of one sweep view whose bins are narrower than 1.75 pixels (at
-// most 3 candidate pixels per bin and line), for the B bins [bx*B, +B) of a block.
+// LDS of the staged forward path: 16 floats per thread of the block (64 B bytes for B
+// threads, 4 KB per wave), shared by every candidate count.
+template <int B>
+constexpr int fwd_stage_floats() { return 16 * B; }
+constexpr int kFwdStagedMaxCand = 6;  // views with more candidates take the gather path
+constexpr int kFwdStageMaxLines = 8;
+
+// Forward projection of one sweep view with at most NC candidate pixels per bin and line
+// (bins narrower than NC - 1.25 pixels), for the B bins [bx*B, +B) of a block.
 // Lines are processed in chunks of FL: for each line of a chunk the block first copies
-// the window of 2B pixels its bins can touch into LDS with 16-byte loads (one contiguous
-// row segment per line; the window starts at the first bin's estimated first pixel
-// minus 2, rounded down to a multiple of 4), then every thread reads its candidates
-// from LDS.  The block's bins span at most (B-1)*1.75 + 1 pixels, so the window always
-// holds them.  Lines run over the union of the block's per-bin line ranges; lines that
-// miss a bin add nothing to it.  Every thread of the block takes part (inactive
+// the window of W = (NC-1)*B pixels its bins can touch into LDS with 16-byte loads (one
+// contiguous row segment per line; the window starts at the first bin's estimated first
+// pixel minus 2, rounded down to a multiple of 4), then every thread reads its
+// candidates from LDS.  The block's bins and their candidates span at most
+// (B-1)*(NC-1.25) + NC + 1 pixels, so the window holds them (offsets are clamped into it
+// all the same).  Lines run over the union of the block's per-bin line ranges; lines
+// that miss a bin add nothing to it.  Every thread of the block takes part (inactive
 // threads, past the last bin, compute a copy of the last bin and store nothing).
-template <int ARITH, int B>
+// sImg holds fwd_stage_floats<B>() floats; sQ0, sQb and sL kFwdStageMaxLines entries.
+template <int ARITH, int B, int NC>
 __device__ __forceinline__ float fwd_staged(const DdView& g, const float* __restrict__ ls,
                                             int npix, int nh, __amdgpu_buffer_rsrc_t rs,
-                                            int bx, int d, float acc)
+                                            int bx, int d, float acc, float* sImg, int* sQ0,
+                                            int* sQb, float* sL)
 {
-    constexpr int W = 2 * B;
-    constexpr int FL = 8;  // 8 x 2B floats = 64B bytes per block: 1 KB per wave, full occupancy
-    __shared__ __attribute__((aligned(16))) float sImg[FL][W];
-    __shared__ int sQ0[FL];
-    __shared__ int sQb[FL];    // 4 * (nh - sQ0): byte offset of pixel index - nh
-    __shared__ float sL[FL];   // L* of the chunk's lines
+    constexpr int W = (NC - 1) * B;
+    constexpr int FL = fwd_stage_floats<B>() / W < kFwdStageMaxLines
+                           ? fwd_stage_floats<B>() / W : kFwdStageMaxLines;
+    constexpr int N4 = FL * (W / 4);          // 16-byte pieces per chunk
+    constexpr int PER = (N4 + B - 1) / B;     // pieces per thread
+    static_assert(FL >= 1 && W % 4 == 0, "staged window shape");
     const int tid = threadIdx.x;
     const int dk = min(d, npix - 1);
     const float Dlo = key_D<ARITH>(g, dk - nh);
@@ -545,16 +555,15 @@ __device__ __forceinline__ float fwd_staged(const DdView& g, const float* __rest
             sL[tid] = ls[c0 + tid];
         }
         __syncthreads();
-        // every thread loads its FL*W/(4B) window pieces into registers first, then
-        // stores them: the loads of one chunk are all in flight together
-        constexpr int PER = FL * (W / 4) / B;
+        // every thread loads its window pieces into registers first, then stores them:
+        // the loads of one chunk are all in flight together
         float4 v[PER];
 #pragma unroll
         for (int r = 0; r < PER; ++r) {
             const int k = tid + r * B;
             const int l = k / (W / 4), j4 = k - l * (W / 4);
             v[r] = make_float4(0.f, 0.f, 0.f, 0.f);
-            if (l < ncl) {
+            if (k < N4 && l < ncl) {
                 const int q = sQ0[l] + 4 * j4;
                 // a window that starts left of the row reads zeros there (negative index
                 // -> out of range); past the row's end it reads the next row: never
@@ -574,7 +583,7 @@ __device__ __forceinline__ float fwd_staged(const DdView& g, const float* __rest
         for (int r = 0; r < PER; ++r) {
             const int k = tid + r * B;
             const int l = k / (W / 4), j4 = k - l * (W / 4);
-            if (l < ncl) *(float4*)&sImg[l][4 * j4] = v[r];
+            if (k < N4 && l < ncl) *(float4*)&sImg[l * W + 4 * j4] = v[r];
         }
         __syncthreads();
 #pragma unroll
@@ -583,14 +592,14 @@ __device__ __forceinline__ float fwd_staged(const DdView& g, const float* __rest
                 const float K = key_K<ARITH>(g, u);
                 const float x = first_pixel(Dlo, K, nh, npix);
                 // byte offset of the first candidate in the line's window
-                const int ob = min(max(((int)x << 2) + sQb[l], 0), 4 * (W - 3));
-                const float* wp = (const float*)((const char*)&sImg[l][0] + ob);
+                const int ob = min(max(((int)x << 2) + sQb[l], 0), 4 * (W - NC));
+                const float* wp = (const float*)((const char*)&sImg[l * W] + ob);
                 // clipping every weight's right end to L* is the exact rule; it changes
                 // only the weights that end at the line's dropped segment
                 const float Dhc = fminf(Dhi, sL[l]);
                 float xk = x, Plo = __fsub_rn(x, K);
 #pragma unroll
-                for (int k = 0; k < 3; ++k)
+                for (int k = 0; k < NC; ++k)
                     acc = cand_step<ARITH>(k == 0, xk, Plo, K, Dlo, Dhc, wp[k], acc);
             }
             u = __fadd_rn(u, 1.0f);
@@ -636,14 +645,27 @@ __device__ __forceinline__ void dd_fwd_block(const DdView* __restrict__ geom,
     double p0 = 0.0;
 
     const bool sweep = g.kind == DD_SWEEP_ROWS || g.kind == DD_SWEEP_COLS;
-    if (PATH != kPathWide && !ONES && sweep && g.ncand <= 3) {
+    if (PATH != kPathWide && !ONES && sweep && g.ncand <= kFwdStagedMaxCand) {
         // block-uniform: every thread joins the LDS-staged path; the epilogue inputs
         // are loaded first so that their latency hides behind the sweep
+        __shared__ __attribute__((aligned(16))) float sImg[fwd_stage_floats<B>()];
+        __shared__ int sQ0[kFwdStageMaxLines], sQb[kFwdStageMaxLines];
+        __shared__ float sL[kFwdStageMaxLines];
         const EpiFwdIn pre = active ? epi_fwd_load<EPI>(o, soff, z, e) : EpiFwdIn{0.0f, 0.0f};
         const __amdgpu_buffer_rsrc_t rs =
             image_rsrc((g.kind == DD_SWEEP_ROWS) ? image : imageT, npix);
         float acc = (EPI == EPI_STORE && accumulate && active) ? out[soff + o] : 0.0f;
-        acc = fwd_staged<ARITH, B>(g, lstar + (int64_t)view * npix, npix, nh, rs, bx, d, acc);
+        const float* ls = lstar + (int64_t)view * npix;
+        // 3, 4 or up to kFwdStagedMaxCand candidates (staging 7-8 measured slower than
+        // the gather path)
+        static_assert(kFwdStagedMaxCand == 6, "staged candidate counts");
+        const int nc = g.ncand;
+        if (nc <= 3)
+            acc = fwd_staged<ARITH, B, 3>(g, ls, npix, nh, rs, bx, d, acc, sImg, sQ0, sQb, sL);
+        else if (nc == 4)
+            acc = fwd_staged<ARITH, B, 4>(g, ls, npix, nh, rs, bx, d, acc, sImg, sQ0, sQb, sL);
+        else
+            acc = fwd_staged<ARITH, B, 6>(g, ls, npix, nh, rs, bx, d, acc, sImg, sQ0, sQb, sL);
         if (active) out[soff + o] = epi_forward<EPI>(acc, pre, &p0);
     } else if (active) {
         float acc = (EPI == EPI_STORE && accumulate) ? out[soff + o] : 0.0f;
