@@ -496,10 +496,14 @@ __device__ __forceinline__ float fwd_grouped(const __amdgpu_buffer_rsrc_t rs, co
 }
 
 // LDS of the staged forward path: 16 floats per thread of the block (64 B bytes for B
-// threads, 4 KB per wave), shared by every candidate count.
+// threads, 4 KB per wave), shared by every candidate count; the launch of the widest
+// views, which runs alone, stages 32 floats per thread.
 template <int B>
 constexpr int fwd_stage_floats() { return 16 * B; }
+template <int B>
+constexpr int fwd_stage_floats_wide() { return 32 * B; }
 constexpr int kFwdStagedMaxCand = 6;  // views with more candidates take the gather path
+constexpr int kFwdStagedWideMaxCand = 32;
 constexpr int kFwdStageMaxLines = 8;
 
 // Forward projection of one sweep view with at most NC candidate pixels per bin and line
@@ -513,16 +517,15 @@ constexpr int kFwdStageMaxLines = 8;
 // all the same).  Lines run over the union of the block's per-bin line ranges; lines
 // that miss a bin add nothing to it.  Every thread of the block takes part (inactive
 // threads, past the last bin, compute a copy of the last bin and store nothing).
-// sImg holds fwd_stage_floats<B>() floats; sQ0, sQb and sL kFwdStageMaxLines entries.
-template <int ARITH, int B, int NC>
+// sImg holds FLOATS floats; sQ0, sQb and sL kFwdStageMaxLines entries.
+template <int ARITH, int B, int NC, int FLOATS>
 __device__ __forceinline__ float fwd_staged(const DdView& g, const float* __restrict__ ls,
                                             int npix, int nh, __amdgpu_buffer_rsrc_t rs,
                                             int bx, int d, float acc, float* sImg, int* sQ0,
                                             int* sQb, float* sL)
 {
     constexpr int W = (NC - 1) * B;
-    constexpr int FL = fwd_stage_floats<B>() / W < kFwdStageMaxLines
-                           ? fwd_stage_floats<B>() / W : kFwdStageMaxLines;
+    constexpr int FL = FLOATS / W < kFwdStageMaxLines ? FLOATS / W : kFwdStageMaxLines;
     constexpr int N4 = FL * (W / 4);          // 16-byte pieces per chunk
     constexpr int PER = (N4 + B - 1) / B;     // pieces per thread
     static_assert(FL >= 1 && W % 4 == 0, "staged window shape");
@@ -645,10 +648,15 @@ __device__ __forceinline__ void dd_fwd_block(const DdView* __restrict__ geom,
     double p0 = 0.0;
 
     const bool sweep = g.kind == DD_SWEEP_ROWS || g.kind == DD_SWEEP_COLS;
-    if (PATH != kPathWide && !ONES && sweep && g.ncand <= kFwdStagedMaxCand) {
+    const bool staged =
+        !ONES && sweep &&
+        (PATH == kPathWide ? g.ncand <= kFwdStagedWideMaxCand : g.ncand <= kFwdStagedMaxCand);
+    if (staged) {
         // block-uniform: every thread joins the LDS-staged path; the epilogue inputs
         // are loaded first so that their latency hides behind the sweep
-        __shared__ __attribute__((aligned(16))) float sImg[fwd_stage_floats<B>()];
+        constexpr int FLOATS =
+            PATH == kPathWide ? fwd_stage_floats_wide<B>() : fwd_stage_floats<B>();
+        __shared__ __attribute__((aligned(16))) float sImg[FLOATS];
         __shared__ int sQ0[kFwdStageMaxLines], sQb[kFwdStageMaxLines];
         __shared__ float sL[kFwdStageMaxLines];
         const EpiFwdIn pre = active ? epi_fwd_load<EPI>(o, soff, z, e) : EpiFwdIn{0.0f, 0.0f};
@@ -656,16 +664,37 @@ __device__ __forceinline__ void dd_fwd_block(const DdView* __restrict__ geom,
             image_rsrc((g.kind == DD_SWEEP_ROWS) ? image : imageT, npix);
         float acc = (EPI == EPI_STORE && accumulate && active) ? out[soff + o] : 0.0f;
         const float* ls = lstar + (int64_t)view * npix;
-        // 3, 4 or up to kFwdStagedMaxCand candidates (staging 7-8 measured slower than
-        // the gather path)
-        static_assert(kFwdStagedMaxCand == 6, "staged candidate counts");
         const int nc = g.ncand;
-        if (nc <= 3)
-            acc = fwd_staged<ARITH, B, 3>(g, ls, npix, nh, rs, bx, d, acc, sImg, sQ0, sQb, sL);
-        else if (nc == 4)
-            acc = fwd_staged<ARITH, B, 4>(g, ls, npix, nh, rs, bx, d, acc, sImg, sQ0, sQb, sL);
-        else
-            acc = fwd_staged<ARITH, B, 6>(g, ls, npix, nh, rs, bx, d, acc, sImg, sQ0, sQb, sL);
+        if constexpr (PATH == kPathWide) {
+            // the launch of views with more than 8 candidates; up to 32 are staged (one
+            // or two lines per chunk), wider views take the gather path
+            static_assert(kFwdStagedWideMaxCand == 32, "staged candidate counts");
+            if (nc <= 12)
+                acc = fwd_staged<ARITH, B, 12, FLOATS>(g, ls, npix, nh, rs, bx, d, acc, sImg,
+                                                       sQ0, sQb, sL);
+            else if (nc <= 16)
+                acc = fwd_staged<ARITH, B, 16, FLOATS>(g, ls, npix, nh, rs, bx, d, acc, sImg,
+                                                       sQ0, sQb, sL);
+            else if (nc <= 24)
+                acc = fwd_staged<ARITH, B, 24, FLOATS>(g, ls, npix, nh, rs, bx, d, acc, sImg,
+                                                       sQ0, sQb, sL);
+            else
+                acc = fwd_staged<ARITH, B, 32, FLOATS>(g, ls, npix, nh, rs, bx, d, acc, sImg,
+                                                       sQ0, sQb, sL);
+        } else {
+            // 3, 4 or up to kFwdStagedMaxCand candidates (staging 7-8 measured slower than
+            // the gather path)
+            static_assert(kFwdStagedMaxCand == 6, "staged candidate counts");
+            if (nc <= 3)
+                acc = fwd_staged<ARITH, B, 3, FLOATS>(g, ls, npix, nh, rs, bx, d, acc, sImg,
+                                                      sQ0, sQb, sL);
+            else if (nc == 4)
+                acc = fwd_staged<ARITH, B, 4, FLOATS>(g, ls, npix, nh, rs, bx, d, acc, sImg,
+                                                      sQ0, sQb, sL);
+            else
+                acc = fwd_staged<ARITH, B, 6, FLOATS>(g, ls, npix, nh, rs, bx, d, acc, sImg,
+                                                      sQ0, sQb, sL);
+        }
         if (active) out[soff + o] = epi_forward<EPI>(acc, pre, &p0);
     } else if (active) {
         float acc = (EPI == EPI_STORE && accumulate) ? out[soff + o] : 0.0f;
