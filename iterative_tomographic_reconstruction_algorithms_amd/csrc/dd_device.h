@@ -738,22 +738,27 @@ __device__ __forceinline__ void dd_fwd_block(const DdView* __restrict__ geom,
             // Up to 16 candidates: lines in groups, all their loads issued before the first
             // is consumed, so a wave keeps several gathers in flight; the lines are then
             // accumulated in increasing order.  Candidates past the bin add nothing.
-            if (fwd_in_path(PATH, 3) && nc <= 3)
+            // Instances whose candidate counts the staged path takes in this launch are
+            // left out (they would only add registers).
+            constexpr int SMAX = ONES ? 0
+                                 : (PATH == kPathWide ? kFwdStagedWideMaxCand
+                                                      : kFwdStagedMaxCand);
+            if (fwd_in_path(PATH, 3) && 3 > SMAX && nc <= 3)
                 acc = fwd_grouped<ARITH, ONES, 3, kFwdLines>(rs, g, ls, i_lo, i_hi, npix, nh, Dlo,
                                                            Dhi, last_bin, acc);
-            else if (fwd_in_path(PATH, 4) && nc == 4)
+            else if (fwd_in_path(PATH, 4) && 4 > SMAX && nc == 4)
                 acc = fwd_grouped<ARITH, ONES, 4, 6>(rs, g, ls, i_lo, i_hi, npix, nh, Dlo, Dhi,
                                                      last_bin, acc);
-            else if (fwd_in_path(PATH, 6) && nc <= 6)
+            else if (fwd_in_path(PATH, 6) && 6 > SMAX && nc <= 6)
                 acc = fwd_grouped<ARITH, ONES, 6, 4>(rs, g, ls, i_lo, i_hi, npix, nh, Dlo, Dhi,
                                                      last_bin, acc);
-            else if (fwd_in_path(PATH, 8) && nc <= 8)
+            else if (fwd_in_path(PATH, 8) && 8 > SMAX && nc <= 8)
                 acc = fwd_grouped<ARITH, ONES, 8, 3>(rs, g, ls, i_lo, i_hi, npix, nh, Dlo, Dhi,
                                                      last_bin, acc);
-            else if (fwd_in_path(PATH, 12) && nc <= 12)
+            else if (fwd_in_path(PATH, 12) && 12 > SMAX && nc <= 12)
                 acc = fwd_grouped<ARITH, ONES, 12, 2>(rs, g, ls, i_lo, i_hi, npix, nh, Dlo, Dhi,
                                                       last_bin, acc);
-            else if (fwd_in_path(PATH, 16) && nc <= 16)
+            else if (fwd_in_path(PATH, 16) && 16 > SMAX && nc <= 16)
                 acc = fwd_grouped<ARITH, ONES, 16, 2>(rs, g, ls, i_lo, i_hi, npix, nh, Dlo, Dhi,
                                                       last_bin, acc);
             else {
