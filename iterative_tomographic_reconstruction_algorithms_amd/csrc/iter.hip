@@ -169,7 +169,7 @@ int dot_f64(const float* a, const float* b, int64_t n, double* out, double* work
 int rd_forward_epi(const Plan& p, int epi, int ones, const float* image, float* out,
                    int nslices, int accumulate, const EpiArgs& e, hipStream_t st,
                    int* blocks_per_slice);
-int rd_back_scatter(const Plan& p, int ones, const float* sino, float* image, int nslices,
+int rd_back_overwrite(const Plan& p, int ones, const float* sino, float* image, int nslices,
                     hipStream_t st);
 
 int projector_forward_epi_any(const Plan& p, int epi, int ones, const float* image, float* out,
@@ -185,7 +185,8 @@ int projector_forward_epi_any(const Plan& p, int epi, int ones, const float* ima
 }
 
 // Back projection followed by a back-side epilogue writing `x`.  The ray-driven path
-// scatters into plan scratch first; the others fuse the epilogue into the gather.
+// projects into plan scratch first and runs the epilogue as its own pass; the others fuse
+// the epilogue into the gather.
 int projector_back_epi_any(const Plan& p, int epi, int ones, const float* sino, float* x,
                            int nslices, const EpiArgs& e, hipStream_t st, int* bps)
 {
@@ -199,7 +200,7 @@ int projector_back_epi_any(const Plan& p, int epi, int ones, const float* sino, 
     int rc = get_scratch(p, nslices, &s, st);
     if (rc) return rc;
     const int64_t n = (int64_t)p.npix * p.npix;
-    rc = rd_back_scatter(p, ones, sino, s.img, nslices, st);
+    rc = rd_back_overwrite(p, ones, sino, s.img, nslices, st);
     if (rc) return rc;
     if (bps) *bps = (int)((n + kEpiChunk - 1) / kEpiChunk);
     return elementwise_back_epilogue(epi, s.img, x, e, n, nslices, st);

@@ -5,14 +5,16 @@
 // since the previous step is credited, with weight 1/5, to the cell just entered, and
 // cells with row or column index 0 are skipped (radon_rd.c:44-66).  The walk repeats
 // the reference build's arithmetic operation for operation (float geometry, double
-// step parameters, double fused update rounded to float after every step), so the
-// forward projection is bit-identical to the reference's.
+// step parameters, double fused update rounded to float after every step).
 //
-// Forward: one thread per (bin, view) walks its five sub-rays in order: no atomics.
-// Back: one thread per (bin, view) walks the same sub-rays and applies each cell update
-// with a compare-and-swap loop that performs exactly the reference's rounding
-// (float(old + 0.2*sino*len), evaluated in double); only the order in which updates of
-// one pixel arrive differs from the reference, so results agree to float rounding.
+// Forward: one thread per (bin, view) walks its five sub-rays in order.
+// Back: one thread per pixel gathers, view by view and sub-ray by sub-ray in increasing
+// index, the one step (if any) of each sub-ray's walk that enters the pixel, and applies
+// it with the reference's rounding.  The walk is a merge of two increasing sequences of
+// crossing parameters (x boundaries and y boundaries), so the step entering a cell, the
+// step before it and whether the walk gets there follow from at most six of those
+// parameters, computed exactly as the walk computes them.  Updates reach each pixel in
+// the reference's own order: both projections are bit-identical to the reference's.
 #include "epilogue.h"
 
 namespace itomo {
@@ -98,38 +100,12 @@ __device__ __forceinline__ float rd_fwd_update(float acc, float pix, float l)
     return (float)__dadd_rn((double)acc, __dmul_rn(__dmul_rn(0.2, (double)l), (double)pix));
 }
 
-__device__ __forceinline__ void cas_update_fast(float* addr, float sv, float l, int arith)
-{
-    unsigned int* a = reinterpret_cast<unsigned int*>(addr);
-    unsigned int old = __hip_atomic_load(a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    for (;;) {
-        const float o = __uint_as_float(old);
-        const float n = arith == ITOMO_ARITH_FAST ? rd_back_update<ITOMO_ARITH_FAST>(o, sv, l)
-                                                  : rd_back_update<ITOMO_ARITH_IEEE>(o, sv, l);
-        const unsigned int prev = atomicCAS(a, old, __float_as_uint(n));
-        if (prev == old) break;
-        old = prev;
-    }
-}
-
-__device__ __forceinline__ void cas_add(float* addr, float v)
-{
-    unsigned int* a = reinterpret_cast<unsigned int*>(addr);
-    unsigned int old = __hip_atomic_load(a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    for (;;) {
-        const float n = __fadd_rn(__uint_as_float(old), v);
-        const unsigned int prev = atomicCAS(a, old, __float_as_uint(n));
-        if (prev == old) break;
-        old = prev;
-    }
-}
-
-// One sub-ray walk (radon_rd.c:15-71).  OPER 0 returns the updated bin value; OPER 1
-// scatters into the image and returns acc unchanged.  The walk is capped at
-// 4*npix+16 steps (a genuine walk takes at most 2*npix+2), so it always ends.
-template <int ARITH, int OPER, bool ONES>
-__device__ float rd_walk(float acc, const float* __restrict__ img_in, float* img_out,
-                         int npix, const float p1[2], const float p2[2])
+// One sub-ray walk of the forward projection (radon_rd.c:15-71): returns the updated
+// bin value.  The walk is capped at 4*npix+16 steps (a genuine walk takes at most
+// 2*npix+2), so it always ends.
+template <int ARITH, bool ONES>
+__device__ float rd_walk(float acc, const float* __restrict__ img_in, int npix,
+                         const float p1[2], const float p2[2])
 {
     const int nh = (int)(npix * 0.5);
     const float d0 = __fsub_rn(p2[0], p1[0]), d1 = __fsub_rn(p2[1], p1[1]);
@@ -148,7 +124,6 @@ __device__ float rd_walk(float acc, const float* __restrict__ img_in, float* img
     float alpha = 0.0f, alpha_old = 0.0f;
     float ax = (float)__ddiv_rn(__dsub_rn(__dadd_rn((double)ii, hx), p1x), dd0);
     float ay = (float)__ddiv_rn(__dsub_rn(__dadd_rn((double)jj, hy), p1y), dd1);
-    const float sv = acc;  // OPER 1: the bin value being back-projected
     const int cap = 4 * npix + 16;
     for (int step = 0; alpha <= 1.0f && step < cap; ++step) {
         if (ax < ay) {
@@ -172,10 +147,7 @@ __device__ float rd_walk(float acc, const float* __restrict__ img_in, float* img
         const int i = ii + nh, j = jj + nh;
         if (i > 0 && i < npix && j > 0 && j < npix) {
             const int64_t k = (int64_t)i * npix + j;
-            if (OPER == 0)
-                acc = rd_fwd_update<ARITH>(acc, ONES ? 1.0f : img_in[k], l);
-            else
-                cas_update_fast(img_out + k, sv, l, ARITH);
+            acc = rd_fwd_update<ARITH>(acc, ONES ? 1.0f : img_in[k], l);
         }
         alpha_old = alpha;
     }
@@ -210,7 +182,7 @@ rd_fwd_kernel(const RdView* __restrict__ geom, int npix, const float* __restrict
             for (int k = 1; k <= 5; ++k) {
                 float p1[2], p2[2];
                 rd_endpoints<ARITH>(6 * b + k, npix, g, p1, p2);
-                acc = rd_walk<ARITH, 0, ONES>(acc, image, nullptr, npix, p1, p2);
+                acc = rd_walk<ARITH, ONES>(acc, image, npix, p1, p2);
             }
         }
         out[soff + o] = epi_forward<EPI>(acc, o, soff, z, e, &p0);
@@ -222,31 +194,103 @@ rd_fwd_kernel(const RdView* __restrict__ geom, int npix, const float* __restrict
     }
 }
 
-template <int ARITH, bool ONES>
-__global__ void __launch_bounds__(64)
-rd_back_kernel(const RdView* __restrict__ geom, int npix, const float* __restrict__ sino,
-               float* __restrict__ image, int64_t sino_stride, int64_t img_stride)
+// Crossing parameter of the walk's m-th boundary along one axis: the boundary between
+// cells c0 + m*dc and c0 + (m+1)*dc, as the walk computes it (radon_rd.c:31-32).
+__device__ __forceinline__ float rd_cross(int c0, int m, int dc, double h, float p, float d)
 {
-    const int b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= npix) return;
-    const int v = blockIdx.y;
-    const RdView g = geom[v];
+    return (float)__ddiv_rn(__dsub_rn(__dadd_rn((double)(c0 + m * dc), h), (double)p),
+                            (double)d);
+}
+
+// The step of one sub-ray's walk that enters cell (ii, jj), if the walk takes it:
+// returns true and the credited length l.  The walk merges the x crossings X(0) < X(1)
+// < ... with the y crossings Y(0) < Y(1) < ... (equal values make one diagonal step);
+// x reaches ii after X(sx-1) and leaves it at X(sx), likewise for y, so the cell is
+// entered at E = max(X(sx-1), Y(sy-1)) if E precedes both X(sx) and Y(sy).  The step
+// before is the larger of the crossings preceding E; the walk takes the step while the
+// one before it is <= 1 (radon_rd.c:28), and credits (E - previous) * |p2 - p1|.
+template <int ARITH>
+__device__ __forceinline__ bool rd_cell_step(const float p1[2], const float p2[2], int ii, int jj,
+                                             float* l_out)
+{
+    const float d0 = __fsub_rn(p2[0], p1[0]), d1 = __fsub_rn(p2[1], p1[1]);
+    const int i0 = (int)rd_round<ARITH>(p1[0]);
+    const int j0 = (int)rd_round<ARITH>(p1[1]);
+    const int dx = (p2[0] > p1[0]) ? 1 : -1, dy = (p2[1] > p1[1]) ? 1 : -1;
+    const int sx = (ii - i0) * dx, sy = (jj - j0) * dy;
+    if (sx < 0 || sy < 0 || (sx == 0 && sy == 0)) return false;  // behind, or the start cell
+    const double hx = 0.5 * (double)dx, hy = 0.5 * (double)dy;
+    const float ninf = __int_as_float(0xff800000);
+    const float Ex = sx > 0 ? rd_cross(i0, sx - 1, dx, hx, p1[0], d0) : ninf;
+    const float Ey = sy > 0 ? rd_cross(j0, sy - 1, dy, hy, p1[1], d1) : ninf;
+    const float E = fmaxf(Ex, Ey);
+    if (!(E < rd_cross(i0, sx, dx, hx, p1[0], d0))) return false;
+    if (!(E < rd_cross(j0, sy, dy, hy, p1[1], d1))) return false;
+    const float Xp = (Ex >= Ey && sx >= 2) ? rd_cross(i0, sx - 2, dx, hx, p1[0], d0) : ninf;
+    const float Yp = (Ey >= Ex && sy >= 2) ? rd_cross(j0, sy - 2, dy, hy, p1[1], d1) : ninf;
+    const float prev = Ex > Ey ? fmaxf(Xp, Ey) : (Ey > Ex ? fmaxf(Yp, Ex) : fmaxf(Xp, Yp));
+    const float aold = (prev == ninf) ? 0.0f : prev;
+    if (!(aold <= 1.0f)) return false;
+    const float incr = (ARITH == ITOMO_ARITH_FAST)
+                           ? (float)__dsqrt_rn((double)__fmaf_rn(d0, d0, __fmul_rn(d1, d1)))
+                           : (float)__dsqrt_rn((double)__fadd_rn(__fmul_rn(d0, d0),
+                                                                 __fmul_rn(d1, d1)));
+    *l_out = __fmul_rn(__fsub_rn(E, aold), incr);
+    return true;
+}
+
+// Back projection as a gather: one thread per pixel (16 x 16 per block), all views in
+// order.  Sub-ray i lies on the line y*c - x*s = tau(i) with tau(i) = nh - i/6, except,
+// for odd npix, sub-rays 6nh+1 and 6nh+2, whose t >= 0 below 3*npix keeps its sign
+// (radon_rd.c:125-128) and puts them on tau = i/6 - nh.  A pixel's cell meets the lines
+// with |tau - tau_c| <= (|c| + |s|)/2, tau_c = jj*c - ii*s; the candidates are those
+// sub-rays plus a third of a pixel of margin for the walk's rounding, and
+// rd_cell_step decides each exactly.
+template <int ARITH, bool ONES>
+__global__ void __launch_bounds__(256)
+rd_back_gather_kernel(const RdView* __restrict__ geom, int nang, int npix,
+                      const float* __restrict__ sino, float* __restrict__ image,
+                      int64_t sino_stride, int64_t img_stride, int accumulate)
+{
+    const int cc = blockIdx.x * 16 + (threadIdx.x & 15);  // image column (y cell)
+    const int r = blockIdx.y * 16 + (threadIdx.x >> 4);   // image row (x cell)
+    if (r >= npix || cc >= npix) return;
     image += (int64_t)blockIdx.z * img_stride;
     sino += (int64_t)blockIdx.z * sino_stride;
-    const float sv = ONES ? 1.0f : sino[(int64_t)v * npix + b];
-    if (g.kind == RD_COLSUM) {
-        const int i = npix - 1 - b;
-        if (i >= 1)
-            for (int j = 0; j < npix; ++j) cas_add(image + (int64_t)j * npix + i, sv);
-    } else if (g.kind == RD_ROWSUM) {
-        for (int j = 0; j < npix; ++j) cas_add(image + (int64_t)b * npix + j, sv);
-    } else {
-        for (int k = 1; k <= 5; ++k) {
-            float p1[2], p2[2];
-            rd_endpoints<ARITH>(6 * b + k, npix, g, p1, p2);
-            (void)rd_walk<ARITH, 1, false>(sv, nullptr, image, npix, p1, p2);
+    const int64_t o = (int64_t)r * npix + cc;
+    float pix = accumulate ? image[o] : 0.0f;
+    const int nh = (int)(npix * 0.5);
+    const int ii = r - nh, jj = cc - nh;
+    const bool inner = r >= 1 && cc >= 1;
+    for (int v = 0; v < nang; ++v) {
+        const RdView g = geom[v];
+        const float* row = sino + (int64_t)v * npix;
+        if (g.kind == RD_COLSUM) {
+            if (cc >= 1) pix = __fadd_rn(pix, ONES ? 1.0f : row[npix - 1 - cc]);
+        } else if (g.kind == RD_ROWSUM) {
+            pix = __fadd_rn(pix, ONES ? 1.0f : row[r]);
+        } else if (inner) {
+            const double sd = (double)g.s, cd = (double)g.c;
+            const double tc = (double)jj * cd - (double)ii * sd;
+            const double hw = 0.5 * (fabs(cd) + fabs(sd));
+            int ilo = (int)floor(6.0 * ((double)nh - tc - hw)) - 2;
+            int ihi = (int)ceil(6.0 * ((double)nh - tc + hw)) + 2;
+            if ((npix & 1) && tc - hw - 1.0 / 3.0 <= 2.0 / 6.0 && tc + hw + 1.0 / 3.0 >= 1.0 / 6.0) {
+                ilo = min(ilo, 6 * nh + 1);
+                ihi = max(ihi, 6 * nh + 2);
+            }
+            ilo = max(ilo, 1);
+            ihi = min(ihi, 6 * npix - 1);
+            for (int i = ilo; i <= ihi; ++i) {
+                if (i % 6 == 0) continue;
+                float p1[2], p2[2], l;
+                rd_endpoints<ARITH>(i, npix, g, p1, p2);
+                if (!rd_cell_step<ARITH>(p1, p2, ii, jj, &l)) continue;
+                pix = rd_back_update<ARITH>(pix, ONES ? 1.0f : row[i / 6], l);
+            }
         }
     }
+    image[o] = pix;
 }
 
 namespace {
@@ -303,23 +347,36 @@ int rd_forward_epi(const Plan& p, int epi, int ones, const float* image, float* 
     return check_hip(hipGetLastError(), "rd forward launch");
 }
 
-// Scatter back projection into `image` (zeroed first).
-int rd_back_scatter(const Plan& p, int ones, const float* sino, float* image, int nslices,
-                    hipStream_t st)
+// Back projection into `image` (overwritten, or accumulated into).
+static int rd_back_launch(const Plan& p, int ones, const float* sino, float* image, int nslices,
+                          int64_t sino_stride, int64_t img_stride, int accumulate,
+                          hipStream_t st)
 {
-    int rc = fill(image, 0.0f, (int64_t)p.npix * p.npix * nslices, st);
-    if (rc) return rc;
-    dim3 block(64), grid((p.npix + 63) / 64, p.nang, nslices);
+    dim3 block(256), grid((p.npix + 15) / 16, (p.npix + 15) / 16, nslices);
     const RdView* g = (const RdView*)p.d_geom;
-    const int64_t ss = (int64_t)p.nang * p.npix, is = (int64_t)p.npix * p.npix;
     if (p.arith == ITOMO_ARITH_FAST) {
-        if (ones) rd_back_kernel<ITOMO_ARITH_FAST, true><<<grid, block, 0, st>>>(g, p.npix, sino, image, ss, is);
-        else rd_back_kernel<ITOMO_ARITH_FAST, false><<<grid, block, 0, st>>>(g, p.npix, sino, image, ss, is);
+        if (ones)
+            rd_back_gather_kernel<ITOMO_ARITH_FAST, true><<<grid, block, 0, st>>>(
+                g, p.nang, p.npix, sino, image, sino_stride, img_stride, accumulate);
+        else
+            rd_back_gather_kernel<ITOMO_ARITH_FAST, false><<<grid, block, 0, st>>>(
+                g, p.nang, p.npix, sino, image, sino_stride, img_stride, accumulate);
     } else {
-        if (ones) rd_back_kernel<ITOMO_ARITH_IEEE, true><<<grid, block, 0, st>>>(g, p.npix, sino, image, ss, is);
-        else rd_back_kernel<ITOMO_ARITH_IEEE, false><<<grid, block, 0, st>>>(g, p.npix, sino, image, ss, is);
+        if (ones)
+            rd_back_gather_kernel<ITOMO_ARITH_IEEE, true><<<grid, block, 0, st>>>(
+                g, p.nang, p.npix, sino, image, sino_stride, img_stride, accumulate);
+        else
+            rd_back_gather_kernel<ITOMO_ARITH_IEEE, false><<<grid, block, 0, st>>>(
+                g, p.nang, p.npix, sino, image, sino_stride, img_stride, accumulate);
     }
     return check_hip(hipGetLastError(), "rd back launch");
+}
+
+int rd_back_overwrite(const Plan& p, int ones, const float* sino, float* image, int nslices,
+                    hipStream_t st)
+{
+    return rd_back_launch(p, ones, sino, image, nslices, (int64_t)p.nang * p.npix,
+                          (int64_t)p.npix * p.npix, 0, st);
 }
 
 int rd_forward(const Plan& p, const float* image, float* sino, int nslices, int64_t,
@@ -329,20 +386,10 @@ int rd_forward(const Plan& p, const float* image, float* sino, int nslices, int6
     return rd_forward_epi(p, EPI_STORE, 0, image, sino, nslices, accumulate, e, st, nullptr);
 }
 
-// Back projection: scatter straight into the output when accumulating, else zero it first.
 int rd_back(const Plan& p, const float* sino, float* image, int nslices, int64_t sino_stride,
             int64_t img_stride, int accumulate, hipStream_t st)
 {
-    if (!accumulate) return rd_back_scatter(p, 0, sino, image, nslices, st);
-    dim3 block(64), grid((p.npix + 63) / 64, p.nang, nslices);
-    const RdView* g = (const RdView*)p.d_geom;
-    if (p.arith == ITOMO_ARITH_FAST)
-        rd_back_kernel<ITOMO_ARITH_FAST, false><<<grid, block, 0, st>>>(g, p.npix, sino, image,
-                                                                        sino_stride, img_stride);
-    else
-        rd_back_kernel<ITOMO_ARITH_IEEE, false><<<grid, block, 0, st>>>(g, p.npix, sino, image,
-                                                                        sino_stride, img_stride);
-    return check_hip(hipGetLastError(), "rd back launch");
+    return rd_back_launch(p, 0, sino, image, nslices, sino_stride, img_stride, accumulate, st);
 }
 
 }  // namespace itomo

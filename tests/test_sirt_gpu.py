@@ -1,11 +1,10 @@
 """GPU parity of the fused SIRT / MLEM steps against the numpy restatement of
 art.py:sirt / sir.py:em driven by the oracle projectors.
 
-Distance-driven, pixel-driven and slant-stacking: the GPU iterates are bitwise equal to the float32 restatement (the
-projectors are, and the fused epilogues perform the same float32 operations in the
-same order).  Ray-driven: the scatter back projection reorders float additions, so the
-iterate after k steps is held to relative L2 <= 1e-5.  FP64 monitors are compared to
-the float64 norms of the restatement's float32 arrays (relative 1e-12).
+For every projector (distance-driven, ray-driven, pixel-driven, slant-stacking) the GPU
+iterates are bitwise equal to the float32 restatement (the projectors are, and the fused
+epilogues perform the same float32 operations in the same order).  FP64 monitors are
+compared to the float64 norms of the restatement's float32 arrays (relative 1e-12).
 """
 import numpy as np
 import pytest
@@ -41,12 +40,8 @@ def test_sirt_matches_restatement(proj, n, v, iters):
     b_np = A(img)
     R, C = plan.sirt_weights()
     R_ref, C_ref = S.sirt_weights(A, At, n, v)
-    if proj in ("dd", "pd", "ss"):
-        np.testing.assert_array_equal(R.cpu().numpy(), R_ref)
-        np.testing.assert_array_equal(C.cpu().numpy(), C_ref)
-    else:
-        np.testing.assert_array_equal(R.cpu().numpy(), R_ref)
-        assert rel_l2(C.cpu().numpy(), C_ref) <= 1e-6
+    np.testing.assert_array_equal(R.cpu().numpy(), R_ref)
+    np.testing.assert_array_equal(C.cpu().numpy(), C_ref)
     b = torch.from_numpy(b_np).cuda()
     x = torch.zeros((n, n), dtype=torch.float32, device="cuda")
     work = torch.empty((v, n), dtype=torch.float32, device="cuda")
@@ -56,12 +51,8 @@ def test_sirt_matches_restatement(proj, n, v, iters):
         plan.sirt_iterate(x, b, R, C, work, pc_beta=0.0, stats=stats)
         res.append(float(stats[0].sqrt()))
     x_ref, info, resid = S.sirt(np.zeros((n, n), np.float32), b_np, A, At, iters, pc=0.0)
-    if proj in ("dd", "pd", "ss"):
-        np.testing.assert_array_equal(x.cpu().numpy(), x_ref)
-        np.testing.assert_allclose(res, resid, rtol=1e-12)
-    else:
-        assert rel_l2(x.cpu().numpy(), x_ref) <= 1e-5
-        np.testing.assert_allclose(res, resid, rtol=1e-5)
+    np.testing.assert_array_equal(x.cpu().numpy(), x_ref)
+    np.testing.assert_allclose(res, resid, rtol=1e-12)
     assert (x.cpu().numpy() >= 0).all() or np.signbit(x.cpu().numpy()[x.cpu().numpy() < 0]).all()
 
 
@@ -97,12 +88,8 @@ def test_mlem_matches_restatement(proj, n, v, iters):
         plan.mlem_iterate(x, b, inv_d, work, stats=stats)
         errs.append(float(stats[0]) / bsq)
     x_ref, err = S.mlem(np.ones((n, n), np.float32), b_np, A, At, iters)
-    if proj in ("dd", "pd", "ss"):
-        np.testing.assert_array_equal(x.cpu().numpy(), x_ref)
-        np.testing.assert_allclose(errs, err, rtol=1e-12)
-    else:
-        assert rel_l2(x.cpu().numpy(), x_ref) <= 1e-5
-        np.testing.assert_allclose(errs, err, rtol=1e-4)
+    np.testing.assert_array_equal(x.cpu().numpy(), x_ref)
+    np.testing.assert_allclose(errs, err, rtol=1e-12)
 
 
 def test_batched_sirt_equals_per_slice():
