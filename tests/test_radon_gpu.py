@@ -184,6 +184,32 @@ def test_full_size_back_single_views_bitwise(full_plan):
         np.testing.assert_array_equal(got, ref, err_msg="view %d" % v)
 
 
+def test_full_size_forward_every_path_bitwise(full_plan):
+    # one view per forward path at 2048^2: LDS-staged with 4 and 6 candidates per bin and
+    # line (views 600, 1500, 755), gathered with 8 (804), staged in the wide launch with
+    # 11, 12, 16, 26 and 32 (1740, 843, 859, 876, 881), gathered in chunks beyond (890)
+    p, ang = full_plan
+    img = np.random.default_rng(7).random((N_FULL, N_FULL)).astype(np.float32)
+    sino = p.forward(gpu(img)).cpu().numpy()
+    for v in (600, 1500, 755, 804, 1740, 843, 859, 876, 881, 890):
+        ref = O.dd_forward(img, np.array([ang[v]], np.float32), O.FAST)
+        np.testing.assert_array_equal(sino[V_FULL - 1 - v], ref[0], err_msg="view %d" % v)
+
+
+def test_full_size_back_single_crossing_threshold_bitwise(full_plan):
+    # the back walk takes its extra-crossing loop only for views whose bin width is within
+    # rounding of the pixel width: at 2048^2 views 36 (93.6 deg) and 936 (3.6 deg) have it,
+    # 37 and 937 just past them do not
+    p, ang = full_plan
+    rng = np.random.default_rng(11)
+    for v in (36, 37, 936, 937, 600, 1740):
+        sino = np.zeros((V_FULL, N_FULL), np.float32)
+        sino[V_FULL - 1 - v] = rng.random(N_FULL).astype(np.float32)
+        got = p.back(gpu(sino)).cpu().numpy()
+        ref = O.dd_back(sino[V_FULL - 1 - v][None, :], np.array([ang[v]], np.float32), O.FAST)
+        np.testing.assert_array_equal(got, ref, err_msg="view %d" % v)
+
+
 def test_full_size_linearity_and_adjointness(full_plan):
     p, _ = full_plan
     x = gpu(ellipses(N_FULL, 40, seed=2))
