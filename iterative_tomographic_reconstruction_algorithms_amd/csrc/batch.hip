@@ -71,6 +71,10 @@ namespace itomo {
 
 __global__ void reset_counter_kernel(int* c) { c[0] = 0; c[1] = 0; }
 
+// Bins per forward work item (one 64-thread block): the per-plan tuner's choice at large
+// sizes; smaller items also balance the persistent grid better than 256-bin ones.
+constexpr int kBatchFwdBlock = 64;
+
 __device__ __forceinline__ int next_item(int* counter)
 {
     __shared__ int s_item;
@@ -81,7 +85,7 @@ __device__ __forceinline__ int next_item(int* counter)
     return it;
 }
 
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(kBatchFwdBlock)
 batch_fwd_kernel(const BatchMember* __restrict__ mem, const BatchItem* __restrict__ items,
                  int nitems, int* counter)
 {
@@ -95,8 +99,8 @@ batch_fwd_kernel(const BatchMember* __restrict__ mem, const BatchItem* __restric
         e.w = m.R;
         e.partials = m.pf;
         e.blocks_per_slice = m.bps_f;
-        const int nbx = (m.npix + kDdFwdBlock - 1) / kDdFwdBlock;
-        dd_fwd_block<ITOMO_ARITH_FAST, EPI_SIRT_FWD, false, kDdFwdBlock>(
+        const int nbx = (m.npix + kBatchFwdBlock - 1) / kBatchFwdBlock;
+        dd_fwd_block<ITOMO_ARITH_FAST, EPI_SIRT_FWD, false, kBatchFwdBlock>(
             m.geom, m.lstar, m.nang, m.npix, m.x, m.xT, m.work, (int64_t)m.npix * m.npix,
             (int64_t)m.nang * m.npix, 0, e, w.a, w.b, w.z, w.b * nbx + w.a);
     }
@@ -210,8 +214,8 @@ int itomo_batch_create(itomo_batch** out, int nmembers, const itomo_plan* const*
         bm.R = R[m];
         bm.C = C[m];
         bm.work = work[m];
-        // the batch launches the default shapes whatever the plan's own choice
-        bm.bps_f = (int)dd_fwd_slots(p->npix, p->nang, kDdFwdBlock);
+        // the batch launches its own shapes whatever the plan's choice
+        bm.bps_f = (int)dd_fwd_slots(p->npix, p->nang, kBatchFwdBlock);
         bm.bps_b = (int)dd_back_blocks(p->npix, kDdBackTile);
         B->h_members.push_back(bm);
         B->plans.push_back(p);
@@ -221,7 +225,7 @@ int itomo_batch_create(itomo_batch** out, int nmembers, const itomo_plan* const*
         scratch_bytes += up(sizeof(double) * 3 * (size_t)bm.bps_f * nslices[m]);
         off_pb[m] = scratch_bytes;
         scratch_bytes += up(sizeof(double) * 3 * (size_t)bm.bps_b * nslices[m]);
-        const int nbf = (p->npix + kDdFwdBlock - 1) / kDdFwdBlock;
+        const int nbf = (p->npix + kBatchFwdBlock - 1) / kBatchFwdBlock;
         const int nbt = (p->npix + kDdBackTile - 1) / kDdBackTile;
         for (int z = 0; z < nslices[m]; ++z) {
             for (int v = 0; v < p->nang; ++v)
@@ -299,8 +303,8 @@ int itomo_batch_sirt_iterate(itomo_batch* B, float pc_beta, void* stream)
         transpose_kernel_b<<<tg, tb, 0, st>>>(bm.x, bm.xT, bm.npix, (int64_t)bm.npix * bm.npix);
     }
     reset_counter_kernel<<<1, 1, 0, st>>>(B->d_counter);
-    batch_fwd_kernel<<<std::min(B->grid, B->nfwd), 256, 0, st>>>(B->d_members, B->d_fwd,
-                                                                   B->nfwd, B->d_counter);
+    batch_fwd_kernel<<<std::min(B->grid * (256 / kBatchFwdBlock), B->nfwd), kBatchFwdBlock, 0,
+                       st>>>(B->d_members, B->d_fwd, B->nfwd, B->d_counter);
     batch_back_kernel<<<std::min(B->grid, B->nback), 256, 0, st>>>(B->d_members, B->d_back,
                                                                      B->nback, B->d_counter,
                                                                      pc_beta);

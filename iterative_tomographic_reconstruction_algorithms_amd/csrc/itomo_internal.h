@@ -111,7 +111,7 @@ constexpr int kNumFwdVariants = 3;
 constexpr int kFwdBins[kNumFwdVariants] = {256, 128, 64};
 constexpr int kNumBackVariants = 3;
 constexpr int kBackTiles[kNumBackVariants][2] = {{64, 4}, {32, 2}, {32, 4}};
-constexpr int kDdFwdBlock = 256;            // default forward shape (also the batch's)
+constexpr int kDdFwdBlock = 256;            // default forward shape
 constexpr int kDdBackTile = 64;             // default back tile (also the batch's)
 constexpr int kRdBlock = 64;
 constexpr int kEpiChunk = 256 * 8;          // pixels per elementwise-epilogue block
