@@ -617,9 +617,10 @@ __device__ __forceinline__ float fwd_staged(const DdView& g, const float* __rest
 // hold consecutive bins, read consecutive addresses.
 // Bins [bx*B, +B) of view `view`, slice z, one per thread; blk indexes the partials.
 // Which views a forward launch handles (block-uniform): all of them, the views with at
-// most 8 candidate pixels per bin and line (kPathMain), or the wider ones (kPathWide).
-// The wide views' deep load groups need many registers; compiled into a kernel of their
-// own they leave the main kernel's occupancy alone.
+// most kFwdMainMaxCand candidate pixels per bin and line (kPathMain), or the wider ones
+// (kPathWide).  The wide views' larger staging windows and deep load groups need LDS and
+// registers; compiled into a kernel of their own they leave the main kernel's occupancy
+// alone.
 constexpr int kPathAll = 0, kPathMain = 1, kPathWide = 2;
 
 // Whether the grouped gather with NC candidates is compiled into launch PATH.
