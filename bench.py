@@ -221,8 +221,9 @@ def main():
                          "(a kernel of known traffic for scaling FETCH_SIZE / WRITE_SIZE)")
     ap.add_argument("--extra-configs", action="store_true",
                     help="also measure the other configurations of the suite (minutes)")
-    ap.add_argument("--max-residual-iters", type=int, default=0,
-                    help="also time plain SIRT from zero to the residual target (0: skip)")
+    ap.add_argument("--max-residual-iters", type=int, default=600,
+                    help="also time plain SIRT from zero to the residual target, the R/C "
+                         "setup included, for at most this many iterations (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (--gpus %d "
@@ -328,6 +329,7 @@ def main():
         x.zero_()
         torch.cuda.synchronize()
         t0 = time.perf_counter()
+        R, C = plan.sirt_weights()
         it, rel = 0, 1.0
         while it < args.max_residual_iters:
             plan.sirt_iterate(x, b, R, C, work, pc_beta=0.0, stats=stats)
