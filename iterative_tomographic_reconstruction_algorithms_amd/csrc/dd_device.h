@@ -430,6 +430,18 @@ __device__ __forceinline__ float first_pixel(float Dlo, float K, int nh, int npi
     return __builtin_amdgcn_fmed3f(fe, (float)(-nh), (float)(npix - 1 - nh));
 }
 
+// first_pixel with one correction instead of two, for views whose keys all lie below 2^20
+// in magnitude (every staged view: bins narrower than 30.75 pixels and npix <= 32766
+// bound |D| and |K| by 16384 * 30.75 < 2^19).  There every rounding of a key or of the
+// estimate is below 1/16, so with q the exact index (P_q <= Dlo < P_q+1),
+// floor(Dlo + K + 1/2) is q or q + 1 and one test against P_fe settles it.
+__device__ __forceinline__ float first_pixel_near(float Dlo, float K, int nh, int npix)
+{
+    float fe = floorf(__fadd_rn(__fadd_rn(Dlo, K), 0.5f));
+    fe = (Dlo < __fsub_rn(fe, K)) ? __fsub_rn(fe, 1.0f) : fe;
+    return __builtin_amdgcn_fmed3f(fe, (float)(-nh), (float)(npix - 1 - nh));
+}
+
 constexpr int kFwdLines = 4;   // lines whose gathers are in flight together
 
 // Most candidate pixels per bin and line of a view in the main forward launch; wider
@@ -593,7 +605,7 @@ __device__ __forceinline__ float fwd_staged(const DdView& g, const float* __rest
         for (int l = 0; l < FL; ++l) {
             if (l < ncl) {
                 const float K = key_K<ARITH>(g, u);
-                const float x = first_pixel(Dlo, K, nh, npix);
+                const float x = first_pixel_near(Dlo, K, nh, npix);
                 // byte offset of the first candidate in the line's window
                 const int ob = min(max(((int)x << 2) + sQb[l], 0), 4 * (W - NC));
                 const float* wp = (const float*)((const char*)&sImg[l * W] + ob);
