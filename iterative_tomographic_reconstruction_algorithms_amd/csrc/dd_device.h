@@ -150,17 +150,14 @@ __device__ __forceinline__ void walk_line(const DdView& g, int npix, int nh, flo
 {
     float x = (float)(p0 - nh);
     float lo = __fsub_rn(x, K);
-    // (index - nh) of the bin holding lo: estimate, one exact correction, then the
-    // clamp to [-1, npix] (bins -1 and npix read staged zeros)
-    float fd = floorf(__fmul_rn(lo, g.est));
-    {
-        const float D0 = key_Df<ARITH>(g, fd);
-        const float D1 = key_Df<ARITH>(g, __fadd_rn(fd, 1.0f));
-        const float up = (lo >= D1) ? 1.0f : 0.0f;
-        const float dn = (lo < D0) ? 1.0f : 0.0f;
-        fd = __fsub_rn(__fadd_rn(fd, up), dn);
-        fd = fminf(fmaxf(fd, (float)(-nh - 1)), (float)(npix - nh));
-    }
+    // (index - nh) of the bin holding lo (D_j <= lo < D_j+1), then the clamp to
+    // [-1, npix] (bins -1 and npix read staged zeros).  In index units the estimate
+    // lo*est is off from lo/(bin width) by a relative 2^-22 (est * rinv = 1 + 2^-23 at
+    // most) and the keys by a relative 2^-24; with |j| <= 2*16384 that stays below 0.01,
+    // so floor(lo*est + 1/2) is j or j + 1 and one test against D_fd settles it.
+    float fd = floorf(__fadd_rn(__fmul_rn(lo, g.est), 0.5f));
+    fd = (lo < key_Df<ARITH>(g, fd)) ? __fsub_rn(fd, 1.0f) : fd;
+    fd = __builtin_amdgcn_fmed3f(fd, (float)(-nh - 1), (float)(npix - nh));
     // fr = (index - nh) of the current bin's right key Dhi; the bin is (int)fr + nh - 1
     // (fr is an integer-valued float, the conversion exact)
     float fr = __fadd_rn(fd, 1.0f);
