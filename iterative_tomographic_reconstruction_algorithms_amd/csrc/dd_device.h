@@ -428,8 +428,9 @@ __device__ __forceinline__ float first_pixel(float Dlo, float K, int nh, int npi
 }
 
 // first_pixel with one correction instead of two, for views whose keys all lie below 2^20
-// in magnitude (every staged view: bins narrower than 30.75 pixels and npix <= 32766
-// bound |D| and |K| by 16384 * 30.75 < 2^19).  There every rounding of a key or of the
+// in magnitude (every view with at most 32 candidates per bin and line, the staged and
+// the grouped paths: bins narrower than 30.75 pixels and npix <= 32766 bound |D| and |K|
+// by 16384 * 30.75 < 2^19; the chunked path for wider views keeps first_pixel).  There every rounding of a key or of the
 // estimate is below 1/16, so with q the exact index (P_q <= Dlo < P_q+1),
 // floor(Dlo + K + 1/2) is q or q + 1 and one test against P_fe settles it.
 __device__ __forceinline__ float first_pixel_near(float Dlo, float K, int nh, int npix)
@@ -463,7 +464,7 @@ __device__ __forceinline__ float fwd_grouped(const __amdgpu_buffer_rsrc_t rs, co
         for (int l = 0; l < GL; ++l) {
             K[l] = key_K<ARITH>(g, u);
             u = __fadd_rn(u, 1.0f);
-            xq[l] = first_pixel(Dlo, K[l], nh, npix);
+            xq[l] = first_pixel_near(Dlo, K[l], nh, npix);
             const int q = (int)xq[l] + nh;
             if (ONES || NC <= kFwdMainMaxCand) {
 #pragma unroll
@@ -485,7 +486,7 @@ __device__ __forceinline__ float fwd_grouped(const __amdgpu_buffer_rsrc_t rs, co
     }
     for (; i <= i_hi; ++i, u = __fadd_rn(u, 1.0f), lbase += npix) {
         const float K = key_K<ARITH>(g, u);
-        const float x = first_pixel(Dlo, K, nh, npix);
+        const float x = first_pixel_near(Dlo, K, nh, npix);
         const int q = (int)x + nh;
         float Dhc = Dhi;
         if (last_bin || Dhi > __fsub_rn(xNm1, K)) Dhc = fminf(Dhi, ls[i]);
