@@ -277,26 +277,32 @@ __device__ __forceinline__ void dd_back_tile(const DdView* __restrict__ geom,
             const DdView g = geom[v0 + v];
             const float* S = &sS[v][0] - sLo[v];
             if (g.kind == DD_SWEEP_ROWS) {
+                if (g.multi) {
 #pragma unroll
-                for (int i = 0; i < P; ++i) {
-                    const float K = key_K<ARITH>(g, line_coord(rr + i, nh));
-                    if (g.multi)
-                        walk_line<ARITH, P, 1, true>(g, npix, nh, K, sLs[v][P * ty + i], cc,
-                                                     S, &acc[i][0]);
-                    else
-                        walk_line<ARITH, P, 1, false>(g, npix, nh, K, sLs[v][P * ty + i], cc,
-                                                      S, &acc[i][0]);
+                    for (int i = 0; i < P; ++i)
+                        walk_line<ARITH, P, 1, true>(g, npix, nh,
+                                                     key_K<ARITH>(g, line_coord(rr + i, nh)),
+                                                     sLs[v][P * ty + i], cc, S, &acc[i][0]);
+                } else {
+#pragma unroll
+                    for (int i = 0; i < P; ++i)
+                        walk_line<ARITH, P, 1, false>(g, npix, nh,
+                                                      key_K<ARITH>(g, line_coord(rr + i, nh)),
+                                                      sLs[v][P * ty + i], cc, S, &acc[i][0]);
                 }
             } else if (g.kind == DD_SWEEP_COLS) {
+                if (g.multi) {
 #pragma unroll
-                for (int j = 0; j < P; ++j) {
-                    const float K = key_K<ARITH>(g, line_coord(cc + j, nh));
-                    if (g.multi)
-                        walk_line<ARITH, P, P, true>(g, npix, nh, K, sLs[v][P * tx + j], rr,
-                                                     S, &acc[0][j]);
-                    else
-                        walk_line<ARITH, P, P, false>(g, npix, nh, K, sLs[v][P * tx + j], rr,
-                                                      S, &acc[0][j]);
+                    for (int j = 0; j < P; ++j)
+                        walk_line<ARITH, P, P, true>(g, npix, nh,
+                                                     key_K<ARITH>(g, line_coord(cc + j, nh)),
+                                                     sLs[v][P * tx + j], rr, S, &acc[0][j]);
+                } else {
+#pragma unroll
+                    for (int j = 0; j < P; ++j)
+                        walk_line<ARITH, P, P, false>(g, npix, nh,
+                                                      key_K<ARITH>(g, line_coord(cc + j, nh)),
+                                                      sLs[v][P * tx + j], rr, S, &acc[0][j]);
                 }
             } else if (g.kind == DD_ROWSUM) {
 #pragma unroll
