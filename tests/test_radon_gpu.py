@@ -92,6 +92,22 @@ def test_accumulate_adds_into_output():
     assert not np.array_equal(out, ref)
 
 
+@pytest.mark.parametrize("arith,flags", [("fast", O.FAST), ("ieee", O.IEEE)])
+def test_rd_back_accumulates_like_the_reference(arith, flags):
+    # the reference adds every update to whatever the image holds: start from a
+    # non-zero image and compare bitwise with the restatement doing the same
+    n, v = 40, 23
+    ang = (angles_deg(v) * np.pi / 180).astype(np.float32)
+    p = Plan("rd", n, ang, arith)
+    rng = np.random.default_rng(9)
+    sino = rng.random((v, n)).astype(np.float32)
+    base = rng.random((n, n)).astype(np.float32)
+    got = p.back(gpu(sino), out=gpu(base), accumulate=True).cpu().numpy()
+    ref = base.copy()
+    O.lib().oracle_radon_rd(ref, n, ang, v, 1, sino.copy(), flags)
+    np.testing.assert_array_equal(got, ref)
+
+
 def test_dd_is_deterministic():
     n, v = 128, 90
     ang = np.fft.fftshift((angles_deg(v) * np.pi / 180).astype(np.float32))
