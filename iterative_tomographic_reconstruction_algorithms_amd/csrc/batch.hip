@@ -283,12 +283,12 @@ int itomo_batch_create(itomo_batch** out, int nmembers, const itomo_plan* const*
 int itomo_batch_destroy(itomo_batch* B)
 {
     if (!B) return ITOMO_OK;
-    hipFree(B->d_members);
-    hipFree(B->d_fwd);
-    hipFree(B->d_back);
-    hipFree(B->d_slices);
-    hipFree(B->d_counter);
-    hipFree(B->d_scratch);
+    (void)hipFree(B->d_members);
+    (void)hipFree(B->d_fwd);
+    (void)hipFree(B->d_back);
+    (void)hipFree(B->d_slices);
+    (void)hipFree(B->d_counter);
+    (void)hipFree(B->d_scratch);
     delete B;
     return ITOMO_OK;
 }
@@ -343,12 +343,12 @@ static int capture(itomo_graph** out, void* stream, int iters,
         rc = rc_enq ? rc_enq : rc_end;
     }
     if (!rc) rc = check_hip(hipGraphInstantiate(&G->exec, G->graph, nullptr, nullptr, 0), "graph: instantiate");
-    if (ev) hipEventDestroy(ev);
-    if (cap) hipStreamDestroy(cap);
+    if (ev) (void)hipEventDestroy(ev);
+    if (cap) (void)hipStreamDestroy(cap);
     if (rc) {
         if (G) {
-            if (G->exec) hipGraphExecDestroy(G->exec);
-            if (G->graph) hipGraphDestroy(G->graph);
+            if (G->exec) (void)hipGraphExecDestroy(G->exec);
+            if (G->graph) (void)hipGraphDestroy(G->graph);
             delete G;
         }
         return rc;
@@ -417,8 +417,8 @@ int itomo_graph_launch(itomo_graph* G, void* stream)
 int itomo_graph_destroy(itomo_graph* G)
 {
     if (!G) return ITOMO_OK;
-    if (G->exec) hipGraphExecDestroy(G->exec);
-    if (G->graph) hipGraphDestroy(G->graph);
+    if (G->exec) (void)hipGraphExecDestroy(G->exec);
+    if (G->graph) (void)hipGraphDestroy(G->graph);
     delete G;
     return ITOMO_OK;
 }
