@@ -8,8 +8,8 @@ to ``mu_max``), the input is the filtered back projection of the sinogram and th
 target is the plain SIRT iterate after ``sirt_iters`` iterations from zero
 (non-negativity on).  The warm start is then x0 = U-Net(FBP(b)); plain SIRT continues
 from x0 and is stopped at the residual ||A x - b|| / ||b|| that plain SIRT from zero
-reached after a given number of iterations.  Plain SIRT stays the fallback and the
-judge of the result: the network only moves the starting point.
+reached after a given number of iterations.  Plain SIRT stays the fallback and decides
+the result: the network only moves the starting point.
 
 Everything runs on the GPU through libitomo; torch allocates and, for the synthetic
 Poisson noise only, draws random numbers.
