@@ -244,8 +244,8 @@ __device__ __forceinline__ bool rd_cell_step(const float p1[2], const float p2[2
 // for odd npix, sub-rays 6nh+1 and 6nh+2, whose t >= 0 below 3*npix keeps its sign
 // (radon_rd.c:125-128) and puts them on tau = i/6 - nh.  A pixel's cell meets the lines
 // with |tau - tau_c| <= (|c| + |s|)/2, tau_c = jj*c - ii*s; the candidates are those
-// sub-rays plus a third of a pixel of margin for the walk's rounding, and
-// rd_cell_step decides each exactly.
+// sub-rays plus one more on each side (a sixth of a pixel, far above the walk's
+// rounding), and rd_cell_step decides each exactly.
 template <int ARITH, bool ONES>
 __global__ void __launch_bounds__(256)
 rd_back_gather_kernel(const RdView* __restrict__ geom, int nang, int npix,
@@ -273,8 +273,8 @@ rd_back_gather_kernel(const RdView* __restrict__ geom, int nang, int npix,
             const double sd = (double)g.s, cd = (double)g.c;
             const double tc = (double)jj * cd - (double)ii * sd;
             const double hw = 0.5 * (fabs(cd) + fabs(sd));
-            int ilo = (int)floor(6.0 * ((double)nh - tc - hw)) - 2;
-            int ihi = (int)ceil(6.0 * ((double)nh - tc + hw)) + 2;
+            int ilo = (int)floor(6.0 * ((double)nh - tc - hw)) - 1;
+            int ihi = (int)ceil(6.0 * ((double)nh - tc + hw)) + 1;
             if ((npix & 1) && tc - hw - 1.0 / 3.0 <= 2.0 / 6.0 && tc + hw + 1.0 / 3.0 >= 1.0 / 6.0) {
                 ilo = min(ilo, 6 * nh + 1);
                 ihi = max(ihi, 6 * nh + 2);
