@@ -125,6 +125,8 @@ __device__ float rd_walk(float acc, const float* __restrict__ img_in, int npix,
     float ax = (float)__ddiv_rn(__dsub_rn(__dadd_rn((double)ii, hx), p1x), dd0);
     float ay = (float)__ddiv_rn(__dsub_rn(__dadd_rn((double)jj, hy), p1y), dd1);
     const int cap = 4 * npix + 16;
+    bool pend = false;  // an update (pv, pl) waits for the next step
+    float pv = 0.0f, pl = 0.0f;
     for (int step = 0; alpha <= 1.0f && step < cap; ++step) {
         if (ax < ay) {
             alpha = ax;
@@ -145,12 +147,17 @@ __device__ float rd_walk(float acc, const float* __restrict__ img_in, int npix,
         }
         const float l = __fmul_rn(__fsub_rn(alpha, alpha_old), incr);
         const int i = ii + nh, j = jj + nh;
-        if (i > 0 && i < npix && j > 0 && j < npix) {
-            const int64_t k = (int64_t)i * npix + j;
-            acc = rd_fwd_update<ARITH>(acc, ONES ? 1.0f : img_in[k], l);
-        }
+        const bool in = i > 0 && i < npix && j > 0 && j < npix;
+        // this step's pixel is loaded now and added one step later, so the load is in
+        // flight while the walk computes the next step; the updates keep their order
+        const float v = (in && !ONES) ? img_in[(int64_t)i * npix + j] : 1.0f;
+        if (pend) acc = rd_fwd_update<ARITH>(acc, pv, pl);
+        pend = in;
+        pv = v;
+        pl = l;
         alpha_old = alpha;
     }
+    if (pend) acc = rd_fwd_update<ARITH>(acc, pv, pl);
     return acc;
 }
 
