@@ -99,7 +99,7 @@ def _events_ms(fn, reps):
 def extra_configs(cpu=True):
     """The other workloads of the benchmark suite, measured on the GPU (opt-in):
     cfg1 ray-driven SIRT 256^2 x 304 x 100 iterations (with the reference C timed on two
-    iterations), a 64-slice sample of the 1024-slice 1024^2 x 900 stack and the mixed
+    iterations), distance-driven MLEM at 2048^2 x 1800 on Poisson counts, a 64-slice sample of the 1024-slice 1024^2 x 900 stack and the mixed
     256-slice launch through the persistent batch (eager and hipGraph), and CG / ADMM-TV
     per-iteration times at 1024^2 x 720 with 1% Gaussian noise."""
     import numpy as np
@@ -148,6 +148,26 @@ def extra_configs(cpu=True):
         cfg1["cpu_kind"] = "reference" if use_ref else "port"
         cfg1["cpu_sample"] = "2 iterations timed single-threaded, x50"
     out["cfg1_rd_sirt_256x304"] = cfg1
+
+    # cfg2 MLEM: Poisson counts I0 * A x / max(A x) with I0 = 1e4, from ones
+    n2, v2 = 2048, 1800
+    p2 = dd_plan(n2, v2)
+    from iterative_tomographic_reconstruction_algorithms_amd import autotune
+    autotune.tune(p2, nslices=1, reps=2, use_cache=False)
+    ax2 = p2.forward(torch.from_numpy(ellipses(n2, 40, seed=2)).to(dev))
+    gen = torch.Generator(device=dev).manual_seed(2)
+    b2 = torch.poisson(1e4 * ax2 / ax2.max(), generator=gen)
+    inv_d2 = p2.mlem_weights()
+    x2 = torch.ones((n2, n2), device=dev)
+    w2 = torch.empty_like(b2)
+    st2 = torch.zeros(3, dtype=torch.float64, device=dev)
+    t_m = _events_ms(lambda: p2.mlem_iterate(x2, b2, inv_d2, w2, stats=st2), 20)
+    out["cfg2_dd_mlem_2048x1800"] = {
+        "workload": "distance-driven MLEM 2048x2048, 1800 views, Poisson counts (I0 = 1e4), x0 = 1",
+        "gpu_ms_per_iter": round(t_m, 4),
+        "gupdates_per_s": round(2.0 * n2 * n2 * v2 / (t_m * 1e-3) / 1e9, 1),
+        "gpu_seconds_200_iters": round(t_m * 200 / 1e3, 3)}
+    del p2, ax2, b2, inv_d2, x2, w2
 
     # cfg3a: 64 of the 1024 slices of a 1024^2 x 900 ellipsoid stack, one persistent batch
     n3, v3, nz3 = 1024, 900, 64
