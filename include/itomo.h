@@ -265,10 +265,15 @@ int itomo_grad_adjoint(const float *g, int npix, int nslices, float *out, void *
 /* Hp += lambd1*p + mu*G^T g (g may be NULL: no TV terms); scal[1] = <p, Hp>. */
 int itomo_cg_hp(float *Hp, const float *p, const float *g, int npix, int nslices,
                 float lambd1, float mu, double *scal, double *work, void *stream);
-/* a = float(scal[0] / (scal[1] + 2^-52)); x += a p; r -= a Hp; scal[2] = <r, r>. */
+/* a = float(scal[0] / (scal[1] + 2^-52)); x += a p; r -= a Hp; scal[2] = <r, r>.
+ * A slice whose scal[0] is negative has stopped (below): x and r stay, scal[2] = 0. */
 int itomo_cg_update(float *x, float *r, const float *p, const float *Hp, int64_t n,
                     int nslices, double *scal, double *work, void *stream);
-/* p = r + float(scal[2] / (scal[0] [+ 2^-52])) p; then scal[0] = scal[2]. */
+/* p = r + float(scal[2] / (scal[0] [+ 2^-52])) p; then scal[0] = scal[2].
+ * The reference's loops stop once ||r|| <= 1e-12 (cg.py:101,128-142, 203,215-231,
+ * 271,274-292): when sqrt(scal[2]) <= 1e-12, p stays and scal[0] becomes -1, which
+ * itomo_cg_update and this call then treat as stopped until a residual call (cg_dot into
+ * slot 0, tv_residual, l1_residual) stores a new scal[0]. */
 int itomo_cg_direction(float *p, const float *r, int64_t n, int nslices, int with_eps,
                        double *scal, void *stream);
 /* scal[slot] = <a, b> per slice. */

@@ -9,12 +9,20 @@
 //   [1] pHp    = <p, H p>
 //   [2] rr_new = <r_new, r_new>
 //   [3] tv     = sum (G x - u)^2  (ADMM multiplier residual)
+// The CG loops of the reference stop once ||r|| <= 1e-12 (cg.py:101,128-142 / 203,215-231 /
+// 271,274-292; only an exactly solved system, such as an all-zero slice, gets there).  The
+// device keeps that decision per slice: after the step that reaches it, rr is set to -1,
+// and a negative rr makes the remaining steps of the loop leave x, r and p alone.  The next
+// residual kernel (cg_dot / tv_residual / l1_residual) writes a fresh rr >= 0.
 #include "epilogue.h"
 
 namespace itomo {
 
 constexpr int kVecBlocks = 256;   // blocks per slice of the vector kernels
 constexpr double kSpacing1 = 2.220446049250313e-16;  // np.spacing(1)
+constexpr double kCgStop = 1e-12;                      // cg.py:101, 203, 271 (eps)
+
+__device__ __forceinline__ bool cg_stops(double rn) { return sqrt(rn) <= kCgStop; }
 
 // ---- np.gradient (edge_order=1, unit spacing) and its adjoint ------------------------
 // Along one axis of length n: interior (f[i+1]-f[i-1])/2, edges f[1]-f[0], f[n-1]-f[n-2].
@@ -132,7 +140,8 @@ cg_update_kernel(float* __restrict__ x, float* __restrict__ r, const float* __re
     p += (int64_t)z * n2;
     Hp += (int64_t)z * n2;
     double s = 0.0;
-    for (int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x; k < n2;
+    // rr < 0: this slice's loop has stopped (x and r stay, <r, r> reads 0)
+    for (int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x; rr >= 0.0 && k < n2;
          k += (int64_t)gridDim.x * 256) {
         x[k] = __fadd_rn(x[k], __fmul_rn(a, p[k]));
         const float rn = __fsub_rn(r[k], __fmul_rn(a, Hp[k]));
@@ -149,6 +158,7 @@ cg_direction_kernel(float* __restrict__ p, const float* __restrict__ r, int64_t 
 {
     const int z = blockIdx.y;
     const double rr = scal[4 * z + 0], rn = scal[4 * z + 2];
+    if (rr < 0.0 || cg_stops(rn)) return;   // stopped before this direction update
     const float bta = (float)(rn / (rr + (with_eps ? kSpacing1 : 0.0)));
     p += (int64_t)z * n2;
     r += (int64_t)z * n2;
@@ -172,10 +182,14 @@ vdot_kernel(const float* __restrict__ a, const float* __restrict__ b, int64_t n,
     block_partials3(s, 0.0, 0.0, partials + ((int64_t)z * kVecBlocks + blockIdx.x) * 3);
 }
 
+// rr <- rr_new, or the stop marker -1 once ||r_new|| <= 1e-12 (kept until the next
+// residual kernel).
 __global__ void scal_shift_kernel(double* scal, int nslices)
 {
     const int z = blockIdx.x * blockDim.x + threadIdx.x;
-    if (z < nslices) scal[4 * z + 0] = scal[4 * z + 2];
+    if (z >= nslices || scal[4 * z + 0] < 0.0) return;
+    const double rn = scal[4 * z + 2];
+    scal[4 * z + 0] = cg_stops(rn) ? -1.0 : rn;
 }
 
 // Copy the finalized sums (stats layout [z*3 + 0]) into scal[4*z + slot].

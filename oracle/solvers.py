@@ -162,10 +162,17 @@ def cg(x0, b, A, At, n_iter, dtype=np.float32, lambd1=0.0, mu=0.0, v=None, tv=Fa
         r = (r - (a * hp).astype(dt)).astype(dt)
         rn = float(np.sum(r.astype(np.float64) ** 2))
         out.append(rn)
+        # cg.py:126-142, 211-231, 274-292: the loops stop once ||r|| <= 1e-12 (only an
+        # exactly solved system gets there; float32 vectors never do)
+        if np.sqrt(rn) <= CG_STOP:
+            break
         bta = dt(rn / (rr + (eps if with_eps else 0.0)))
         p = (r + (bta * p).astype(dt)).astype(dt)
         rr = rn
     return x, np.array(out)
+
+
+CG_STOP = 1e-12     # cg.py:101, 203, 271: eps of the inner CG loops
 
 
 def admm_l1(x0, b, A, At, n_iter, n_inner=4, lambd1=1.0, lambd2=1.0, mu=1.0,
@@ -194,10 +201,16 @@ def soft(y, t):
     return np.maximum(0, y - t) - np.maximum(0, -y - t)
 
 
-def admm_tv(x0, b, A, At, n_iter, n_inner=4, lambd1=1.0, lambd2=1.0, mu=1.0, dtype=np.float32):
-    """admm.py:470-604 for one slice: inner CG (cg.py:256-310) then
-    u = soft(G x + alpha/mu, lambd2/mu), alpha += G x - u.  Returns (x, err) with
-    err[k] = ||G x - u|| after outer step k."""
+def admm_tv(x0, b, A, At, n_iter, n_inner=4, lambd1=1.0, lambd2=1.0, mu=1.0, dtype=np.float32,
+            return_state=False):
+    """admm.py:470-604 for one slice: inner CG (cg.py:256-310) on
+    (At A + lambd1 I + mu Gt G) x = At b + Gt v with v = mu u - alpha (admm.py:532-534),
+    then u = soft(G x + alpha/mu, lambd2/mu) (admm.py:460-461, 565-571) and
+    alpha += G x - u (admm.py:464-465, 576-593).  Returns (x, err) with
+    err[k] = ||G x - u|| after outer step k, and (x, err, u, alpha) when return_state.
+
+    The three steps are the module functions admm_tv_rhs / admm_tv_shrink /
+    admm_tv_dual, looked up at call time."""
     dt = dtype
     n = x0.shape[-1]
     x = x0.astype(dt).copy()
@@ -205,15 +218,33 @@ def admm_tv(x0, b, A, At, n_iter, n_inner=4, lambd1=1.0, lambd2=1.0, mu=1.0, dty
     al = np.zeros((2, n, n), dtype=dt)
     err = []
     for _ in range(n_iter):
-        v = (dt(mu) * u - al).astype(dt)
+        v = admm_tv_rhs(u, al, mu, dt)
         x, _ = cg(x, b, A, At, n_inner, dtype=dt, lambd1=lambd1, mu=mu, v=v, tv=True,
                   with_eps=False)
         g = np.array([gg.astype(dt) for gg in gradient(x)])
-        u = soft((g + (al / dt(mu)).astype(dt)).astype(dt), dt(lambd2) / dt(mu)).astype(dt)
-        r = (g - u).astype(dt)
-        al = (al + r).astype(dt)
+        u = admm_tv_shrink(g, al, mu, lambd2, dt)
+        al, r = admm_tv_dual(al, g, u, dt)
         err.append(np.sqrt(np.sum(r.astype(np.float64) ** 2)))
+    if return_state:
+        return x, np.array(err), u, al
     return x, np.array(err)
+
+
+def admm_tv_rhs(u, al, mu, dt):
+    """v = mu*u - alpha, the TV data term of the inner CG (admm.py:532-534)."""
+    return (dt(mu) * u - al).astype(dt)
+
+
+def admm_tv_shrink(g, al, mu, lambd2, dt):
+    """u = soft(G x + alpha/mu, lambd2/mu) (admm.py:460-461)."""
+    return soft((g + (al / dt(mu)).astype(dt)).astype(dt), dt(lambd2) / dt(mu)).astype(dt)
+
+
+def admm_tv_dual(al, g, u, dt):
+    """alpha += G x - u, without a factor mu (admm.py:464-465, 576-593).  Returns
+    (alpha, G x - u)."""
+    r = (g - u).astype(dt)
+    return (al + r).astype(dt), r
 
 
 def calc_filter(nfreq, ftype="ramp"):

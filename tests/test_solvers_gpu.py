@@ -136,3 +136,25 @@ def test_dropin_solvers_numpy_interface():
     pa = param(npix_op=n, nang=v, n_iter=2, reg="lasso-tv", projector="dist-driv")
     xa, erra = admm_mod.admm(b, deg, pa)
     assert xa.shape == (1, n, n) and len(erra) == 2 and np.isfinite(xa).all()
+
+
+@pytest.mark.parametrize("which", ["tv", "l1"])
+def test_admm_inner_cg_stops_on_an_exactly_solved_slice(which):
+    # a stack with an all-zero slice (x0 = 0, b = 0): its inner CG residual is 0, the
+    # reference's loops stop at ||r|| <= 1e-12 (cg.py:203,271) and the slice stays 0;
+    # without the stop the direction update divides 0 by 0.  The other slice is unaffected.
+    n, v, it = 32, 30, 3
+    plan, A, At, A64, At64 = geometry(n, v)
+    b1 = A(ellipses(n, 6, seed=8))
+    b = np.stack([np.zeros_like(b1), b1])
+    x0 = np.stack([np.zeros((n, n), np.float32), np.ones((n, n), np.float32)])
+    run = G.admm_tv if which == "tv" else G.admm_l1
+    ref = S.admm_tv if which == "tv" else S.admm_l1
+    x, err = run(plan, dev(x0), dev(b), it, n_inner=4, lambd1=0.5, lambd2=0.2, mu=1.0)
+    x = x.cpu().numpy()
+    assert np.isfinite(x).all()
+    np.testing.assert_array_equal(x[0], 0.0)
+    x_ref, _ = ref(np.zeros((n, n), np.float32), np.zeros_like(b1), A, At, it, 4, 0.5, 0.2, 1.0)
+    np.testing.assert_array_equal(x_ref, 0.0)
+    x1_ref, _ = ref(np.ones((n, n), np.float32), b1, A, At, it, 4, 0.5, 0.2, 1.0)
+    assert rel(x[1], x1_ref) <= 1e-4

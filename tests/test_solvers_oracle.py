@@ -168,3 +168,76 @@ def test_admm_l1_reaches_the_l1_regularised_minimiser():
                        mu=1.0, dtype=np.float64)
     np.testing.assert_allclose(x, S.soft(b, 0.3) / 1.5, atol=1e-6)
     assert err[-1] < 1e-6
+
+
+# ---- filter windows (filters.py:35-76), pinned by properties the formulas do not repeat --
+# With L = 2 nfreq samples, filtarr = k/nfreq = w/pi on k <= nfreq (w = pi k / nfreq) and
+# the second half mirrors the first.  So the windows are H(w) = |w|/pi times
+# 1 (ramp), pi sin(w)/(4 w) (shepp-logan), pi cos(w)/(2 w) (cosine),
+# 0.54 + 0.46 cos w (hamming), 1 + cos(w)/2 (hanning), with H(0) = 0, and their L-periodic
+# impulse responses follow from the ramp's by the shift theorem (cos w = (e^iw + e^-iw)/2).
+
+NYQUIST = {"ramp": 1.0, "shepp-logan": 0.0, "cosine": -1.0 / (2 * np.pi), "hamming": 0.08,
+           "hanning": 0.5}
+
+
+@pytest.mark.parametrize("ftype", sorted(NYQUIST))
+def test_filter_window_dc_nyquist_and_symmetry(ftype):
+    for nfreq in (16, 128, 1024):
+        H = S.calc_filter(nfreq, ftype)
+        assert H.shape == (2 * nfreq,)
+        assert H[0] == 0.0                                   # no DC gain
+        assert abs(H[nfreq] - NYQUIST[ftype]) < 1e-15        # value at Nyquist (w = pi)
+        np.testing.assert_array_equal(H[1:nfreq], H[:nfreq:-1])   # real, even filter
+
+
+def _impulse(nfreq, ftype):
+    h = np.fft.ifft(S.calc_filter(nfreq, ftype))
+    assert np.max(np.abs(h.imag)) < 1e-15
+    return h.real
+
+
+@pytest.mark.parametrize("ftype,a,c", [("hamming", 0.54, 0.23), ("hanning", 1.0, 0.25)])
+def test_raised_cosine_windows_are_shifted_ramps(ftype, a, c):
+    # H = ramp * (a + 2c cos w)  =>  h[n] = a r[n] + c (r[n-1] + r[n+1]), r the ramp's taps
+    r = _impulse(128, "ramp")
+    h = _impulse(128, ftype)
+    np.testing.assert_allclose(h, a * r + c * (np.roll(r, 1) + np.roll(r, -1)), rtol=0,
+                               atol=1e-15)
+
+
+def test_cosine_window_impulse_closed_form():
+    # H(k) = cos(2 pi k / L) / (2 pi) for k != 0 and H(0) = 0, so
+    # h[n] = (delta[n-1] + delta[n+1]) / (4 pi) - 1 / (2 pi L)
+    nfreq = 128
+    L = 2 * nfreq
+    n = np.arange(L)
+    ref = np.where((n == 1) | (n == L - 1), 1.0 / (4 * np.pi), 0.0) - 1.0 / (2 * np.pi * L)
+    np.testing.assert_allclose(_impulse(nfreq, "cosine"), ref, rtol=0, atol=1e-15)
+
+
+def test_shepp_logan_window_impulse_limit():
+    # H(w) = |sin w| / (4 pi): h[n] -> (1/(4 pi^2)) int_0^pi sin w cos(n w) dw
+    #      = (1 + (-1)^n) / (4 pi^2 (1 - n^2)) (0 for odd n), up to O(1/L^2)
+    h = _impulse(128, "shepp-logan")
+    n = np.arange(12)
+    lim = np.zeros(12)
+    ev = n % 2 == 0
+    lim[ev] = 2.0 / (4 * np.pi ** 2 * (1.0 - n[ev].astype(float) ** 2))
+    np.testing.assert_allclose(h[:12], lim, rtol=0, atol=1e-5)
+    assert np.max(np.abs(h[1:12:2])) < 1e-15
+
+
+@pytest.mark.parametrize("ftype", sorted(NYQUIST) + ["none"])
+def test_library_taps_are_the_windows_impulse_response(ftype):
+    # itomo_filter_taps (a host function of the library, no GPU) returns the first npix
+    # taps of the L-periodic impulse response of the restated window
+    import ctypes
+    from iterative_tomographic_reconstruction_algorithms_amd import _lib
+    for npix in (37, 64, 100):
+        nfreq = 2 * int(2 ** int(np.ceil(np.log2(npix))))
+        h = np.empty(npix)
+        assert _lib.lib().itomo_filter_taps(npix, _lib.FILTERS[ftype],
+                                            h.ctypes.data_as(ctypes.c_void_p)) == 0
+        ref = np.fft.ifft(S.calc_filter(nfreq, ftype)).real[:npix]
+        np.testing.assert_allclose(h, ref, rtol=0, atol=1e-12)
