@@ -6,6 +6,14 @@
  * hipStream_t passed as void* (NULL = the null stream).  Calls are stream-ordered and
  * asynchronous unless stated; they allocate no device memory unless stated.
  *
+ * Plan scratch.  A plan owns one device scratch buffer (the transposed image of the
+ * distance-driven forward, FP64 partial sums, the ray-driven back projection).  The
+ * projection and iteration calls that take a plan (itomo_forward/back, itomo_sirt_*,
+ * itomo_mlem_*, itomo_sps_*) size it for (plan, nslices, launch shapes) on first use and
+ * whenever a call needs more: that growth synchronizes `stream`, frees the old buffer and
+ * allocates the new one.  Calls that need no more than the buffer holds allocate
+ * nothing.  A plan must not be used from two streams at once.
+ *
  * Array layouts are the reference's:
  *   image : npix x npix, row-major (img[row][col])
  *   sino  : nang x npix; for the distance-driven projector row nang-1-v holds view v of
@@ -195,6 +203,11 @@ int itomo_batch_destroy(itomo_batch *batch);
  * hipGraph capture of `iters` SIRT iterations (single plan or batch); the graph is
  * replayed with itomo_graph_launch on any stream.  Capture synchronizes once with the
  * work already queued on `stream`.  The graph reuses the arrays it was captured with.
+ * itomo_graph_sirt allocates the graph's own scratch (the plan-scratch layout for
+ * nslices, see above) before capture and itomo_graph_destroy frees it, so later calls
+ * on the plan (more slices, other launch shapes) never touch memory a replay uses.  The
+ * plan and the captured arrays must outlive the graph; destroy it only after its
+ * replays have finished.  Batch graphs use the batch's own scratch.
  * ------------------------------------------------------------------------------- */
 typedef struct itomo_graph itomo_graph;
 int itomo_graph_sirt(itomo_graph **graph, const itomo_plan *plan, float *x, const float *b,

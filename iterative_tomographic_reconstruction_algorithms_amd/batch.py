@@ -10,13 +10,23 @@ import ctypes
 import torch
 
 from . import _lib as L
-from .core import Plan, _p, device, stream_handle
+from .core import Plan, _check_dev, _p, device, stream_handle
 
 
 class Member:
-    """One geometry of a batch: a plan and its stacked slices, all on the GPU."""
+    """One geometry of a batch: a plan and its stacked slices, all on the GPU.
+    ``b`` holds the slices' sinograms, shape (V, N) or (nz, V, N) of the plan."""
 
     def __init__(self, plan, b):
+        # the persistent kernels read and write nang*npix entries per slice of b
+        _check_dev(b, "b")
+        if b.dim() not in (2, 3) or tuple(b.shape[-2:]) != (plan.nang, plan.npix):
+            raise ValueError("b has shape %s; a member of this plan needs (%d, %d) or "
+                             "(nz, %d, %d)" % (tuple(b.shape), plan.nang, plan.npix,
+                                               plan.nang, plan.npix))
+        if plan.oper != "dd" or plan.arith != "fast":
+            raise ValueError("batch members are distance-driven plans with the reference "
+                             "build's arithmetic (got %s/%s)" % (plan.oper, plan.arith))
         self.plan = plan
         self.b = b if b.dim() == 3 else b.unsqueeze(0)
         nz = self.b.shape[0]

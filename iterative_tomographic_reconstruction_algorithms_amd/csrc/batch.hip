@@ -65,6 +65,7 @@ struct itomo_batch {
 struct itomo_graph {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    void* scratch = nullptr;   // the graph's own scratch region (single-plan graphs)
 };
 
 namespace itomo {
@@ -317,10 +318,15 @@ int itomo_batch_sirt_iterate(itomo_batch* B, float pc_beta, void* stream)
 int itomo_sirt_iterate(const itomo_plan*, float*, const float*, const float*, const float*,
                        float*, int, float, double*, void*);
 
+// Captures `iters` calls of enqueue(ctx) into a graph.  `scratch` (may be null) is a
+// device region owned by the new graph: freed by itomo_graph_destroy, or here on failure.
 static int capture(itomo_graph** out, void* stream, int iters,
-                   int (*enqueue)(void* ctx, hipStream_t st), void* ctx)
+                   int (*enqueue)(void* ctx, hipStream_t st), void* ctx, void* scratch)
 {
-    if (!out || iters < 1) return set_error(ITOMO_EINVAL, "graph: bad argument");
+    if (!out || iters < 1) {
+        if (scratch) (void)hipFree(scratch);
+        return set_error(ITOMO_EINVAL, "graph: bad argument");
+    }
     *out = nullptr;
     hipStream_t st = (hipStream_t)stream, cap = nullptr;
     int rc = check_hip(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking), "graph: stream");
@@ -351,8 +357,10 @@ static int capture(itomo_graph** out, void* stream, int iters,
             if (G->graph) (void)hipGraphDestroy(G->graph);
             delete G;
         }
+        if (scratch) (void)hipFree(scratch);
         return rc;
     }
+    G->scratch = scratch;
     *out = G;
     return ITOMO_OK;
 }
@@ -381,12 +389,18 @@ int itomo_graph_sirt(itomo_graph** out, const itomo_plan* p, float* x, const flo
                      double* stats, int iters, void* stream)
 {
     if (!p) return set_error(ITOMO_EINVAL, "graph_sirt: null plan");
-    // plan scratch must exist before capture (no allocation inside a captured region)
-    Scratch s;
-    int rc = get_scratch(*p, nslices, &s, (hipStream_t)stream);
+    if (nslices < 1 || nslices > 65535)
+        return set_error(ITOMO_EINVAL, "graph_sirt: nslices %d outside [1, 65535]", nslices);
+    // The graph gets its own scratch, allocated before capture (nothing is allocated
+    // inside a captured region) and freed with the graph: later calls on the plan may
+    // grow or re-lay out the plan's own buffer without touching what the graph replays.
+    const size_t bytes = scratch_total(*p, nslices);
+    void* scratch = nullptr;
+    int rc = check_hip(hipMalloc(&scratch, bytes), "graph_sirt: hipMalloc scratch");
     if (rc) return rc;
+    ScratchScope scope(ScratchRegion{scratch, bytes});
     SirtCtx ctx{p, x, b, R, C, work, nslices, pc_beta, stats};
-    return capture(out, stream, iters, enqueue_sirt, &ctx);
+    return capture(out, stream, iters, enqueue_sirt, &ctx, scratch);
 }
 
 struct BatchCtx {
@@ -405,7 +419,7 @@ int itomo_graph_batch_sirt(itomo_graph** out, itomo_batch* B, float pc_beta, int
 {
     if (!B) return set_error(ITOMO_EINVAL, "graph_batch_sirt: null batch");
     BatchCtx ctx{B, pc_beta};
-    return capture(out, stream, iters, enqueue_batch, &ctx);
+    return capture(out, stream, iters, enqueue_batch, &ctx, nullptr);
 }
 
 int itomo_graph_launch(itomo_graph* G, void* stream)
@@ -419,6 +433,7 @@ int itomo_graph_destroy(itomo_graph* G)
     if (!G) return ITOMO_OK;
     if (G->exec) (void)hipGraphExecDestroy(G->exec);
     if (G->graph) (void)hipGraphDestroy(G->graph);
+    if (G->scratch) (void)hipFree(G->scratch);
     delete G;
     return ITOMO_OK;
 }

@@ -167,6 +167,26 @@ struct Scratch {
     float* img;    // nslices images (ray-driven scatter target)
 };
 int get_scratch(const Plan& p, int nslices, Scratch* s, hipStream_t st);
+// Bytes of scratch get_scratch carves for (plan, nslices) with the plan's current shapes.
+size_t scratch_total(const Plan& p, int nslices);
+
+// A caller-owned device region that get_scratch uses instead of the plan's buffer while
+// it is set on the calling thread (graph capture gives each graph its own region).  A
+// call whose layout does not fit fails with ITOMO_EINVAL and allocates nothing.
+struct ScratchRegion {
+    void* base;
+    size_t bytes;
+};
+// Installs `r` on this thread for the lifetime of the guard.
+class ScratchScope {
+public:
+    explicit ScratchScope(ScratchRegion r);
+    ~ScratchScope();
+    ScratchScope(const ScratchScope&) = delete;
+    ScratchScope& operator=(const ScratchScope&) = delete;
+private:
+    ScratchRegion prev_;
+};
 
 // error reporting
 int set_error(int status, const char* fmt, ...);

@@ -242,7 +242,13 @@ int plan_scratch(const Plan& p, size_t bytes, void** ptr, hipStream_t st)
     return ITOMO_OK;
 }
 
-int get_scratch(const Plan& p, int nslices, Scratch* s, hipStream_t st)
+namespace {
+
+struct ScratchLayout {
+    size_t o_pf, o_pb, o_img, total;
+};
+
+ScratchLayout scratch_layout(const Plan& p, int nslices)
 {
     const size_t ns = (size_t)nslices, n2 = (size_t)p.npix * p.npix;
     const size_t nxt = n2 * ns;
@@ -252,18 +258,42 @@ int get_scratch(const Plan& p, int nslices, Scratch* s, hipStream_t st)
     const size_t nb = (nbb > nbe ? nbb : nbe) * 3 * ns;
     const size_t ni = p.projector == ITOMO_PROJ_RD ? n2 * ns : 0;
     auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
-    const size_t o_pf = up(nxt * sizeof(float));
-    const size_t o_pb = o_pf + up(nf * sizeof(double));
-    const size_t o_img = o_pb + up(nb * sizeof(double));
-    const size_t total = o_img + up(ni * sizeof(float));
+    ScratchLayout l;
+    l.o_pf = up(nxt * sizeof(float));
+    l.o_pb = l.o_pf + up(nf * sizeof(double));
+    l.o_img = l.o_pb + up(nb * sizeof(double));
+    l.total = l.o_img + up(ni * sizeof(float));
+    return l;
+}
+
+thread_local ScratchRegion t_region = {nullptr, 0};
+
+}  // namespace
+
+ScratchScope::ScratchScope(ScratchRegion r) : prev_(t_region) { t_region = r; }
+ScratchScope::~ScratchScope() { t_region = prev_; }
+
+size_t scratch_total(const Plan& p, int nslices) { return scratch_layout(p, nslices).total; }
+
+int get_scratch(const Plan& p, int nslices, Scratch* s, hipStream_t st)
+{
+    const ScratchLayout l = scratch_layout(p, nslices);
     void* base = nullptr;
-    int rc = plan_scratch(p, total, &base, st);
-    if (rc) return rc;
+    if (t_region.base) {
+        if (l.total > t_region.bytes)
+            return set_error(ITOMO_EINVAL, "scratch: %zu bytes needed for %d slice(s) of %dx%d, "
+                                           "the supplied region holds %zu",
+                             l.total, nslices, p.npix, p.npix, t_region.bytes);
+        base = t_region.base;
+    } else {
+        int rc = plan_scratch(p, l.total, &base, st);
+        if (rc) return rc;
+    }
     char* b = (char*)base;
     s->xT = (float*)b;
-    s->pf = (double*)(b + o_pf);
-    s->pb = (double*)(b + o_pb);
-    s->img = (float*)(b + o_img);
+    s->pf = (double*)(b + l.o_pf);
+    s->pb = (double*)(b + l.o_pb);
+    s->img = (float*)(b + l.o_img);
     return ITOMO_OK;
 }
 

@@ -46,11 +46,13 @@ def sps(x, b, tp, param):
         raise ValueError("unknown SPS penalty %r" % (reg,))
     n_iter = int(param.n_iter) if param.n_iter is not None and param.n_iter < 1e19 else 200
     clip = not bool(getattr(param, "edge_padding", False))
+    eps = -1.0 if getattr(param, "eps", None) is None else float(param.eps)
     xd = _dev(x).clone()
     plan = tp.plan(xd.shape[-1])
     xd, err = solvers.sps(plan, xd, _dev(b), n_iter, penalty=reg,
                           beta=float(getattr(param, "reg_cost", 0.0) or 0.0),
-                          delta=float(getattr(param, "huber_cost", 0.1) or 0.1), clip=clip)
+                          delta=float(getattr(param, "huber_cost", 0.1) or 0.1), clip=clip,
+                          eps=eps)
     xd = xd.reshape(x.shape) if isinstance(x, torch.Tensor) else xd.reshape(np.shape(x))
     return _out(xd, x), np.stack([err, np.zeros_like(err)], axis=1)
 

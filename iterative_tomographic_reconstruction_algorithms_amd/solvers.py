@@ -93,10 +93,11 @@ def mlem(plan, x, b, n_iter, eps=-1.0):
     hist = torch.zeros((n_iter + 1, 3 * nz), dtype=torch.float64, device=x.device)
     for it in range(n_iter):
         plan.mlem_iterate(x, b, inv_d, work, stats=hist[it])
-        if eps > 0 and (it + 1) % 10 == 0:
-            if float(hist[it, 3 * nzz]) / bsq <= eps:
-                n_iter = it + 1
-                break
+        # sir.py:256,287: `while it < n_iter and err > eps` ends the loop right after the
+        # first step whose err (of the iterate entering it) is <= eps
+        if eps >= 0 and not float(hist[it, 3 * nzz]) / bsq > eps:
+            n_iter = it + 1
+            break
     # residual of the final iterate, without changing x: forward epilogue only
     fin = torch.zeros(3 * nz, dtype=torch.float64, device=x.device)
     plan.sirt_forward(x, b, torch.ones_like(b[0]), work, stats=fin)
@@ -109,15 +110,16 @@ def mlem(plan, x, b, n_iter, eps=-1.0):
 
 # ---- SPS ------------------------------------------------------------------------------
 
-def sps(plan, x, b, n_iter, penalty=None, beta=0.0, delta=0.1, clip=True):
+def sps(plan, x, b, n_iter, penalty=None, beta=0.0, delta=0.1, clip=True, eps=-1.0):
     """sir.py:348-560: separable paraboloidal surrogate with w = exp(-b/max b),
     den = A^T(w * A(1)) and the FP64 quotient in the update
     x <- x - (A^T(w (A x - b)) + beta R') / (den + beta R'' + eps).  ``penalty`` is None,
     "huber" (on the Sobel magnitude of x, threshold ``delta``), "tikhonov" or "haar"
     (sir.py:455-530); ``clip`` applies np.clip(x, 0, max x) after each step, the
     reference's default when there is no edge padding (sir.py:539-540).
-    Returns (x, err) with err[k] = sum ((A x_k - b) w)^2 / sum b^2 over all slices
-    (sir.py:546)."""
+    Returns (x, err) with err[k] = sum ((A x_k - b) w)^2 over the middle slice / sum b^2
+    over all slices (sir.py:366, 546).  With ``eps`` >= 0 the loop stops right after the
+    first step whose err is <= eps (sir.py:426)."""
     x = _as3(x)
     b = _as3(b)
     nz = x.shape[0]
@@ -130,15 +132,19 @@ def sps(plan, x, b, n_iter, penalty=None, beta=0.0, delta=0.1, clip=True):
     kind = L.PENALTY[penalty]
     img_work = torch.empty(((4 if kind else 1) * nz,) + tuple(x.shape[-2:]),
                            dtype=torch.float32, device=x.device)
+    nzz = 0 if nz == 1 else int(nz * 0.5)
     hist = torch.zeros((n_iter, 3 * nz), dtype=torch.float64, device=x.device)
+    bden = float(torch.sum(b.double() ** 2))
     for it in range(n_iter):
         L.check(L.lib().itomo_sps_step(plan._h, _p(x), _p(b), _p(w), _p(den), _p(work),
                                        _p(img_work), nz, kind, float(beta), float(delta),
                                        int(bool(clip)), _p(hist[it]), stream_handle()),
                 "sps_step")
-    bden = float(torch.sum(b.double() ** 2))
-    h = hist.cpu().numpy().reshape(n_iter, nz, 3)
-    return x, h[:, :, 0].sum(axis=1) / bden
+        if eps >= 0 and not float(hist[it, 3 * nzz]) / bden > eps:
+            n_iter = it + 1
+            break
+    h = hist[:n_iter].cpu().numpy().reshape(n_iter, nz, 3)
+    return x, h[:, nzz, 0] / bden
 
 
 def penalty(kind, image, delta=0.1, arith="fast"):

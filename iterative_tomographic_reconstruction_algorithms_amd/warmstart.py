@@ -28,7 +28,7 @@ from .phantom import angles_deg, ellipses, textured
 from .unet import UNet
 
 __all__ = ["dd_plan", "fbp", "sirt", "residual", "make_pairs", "train", "iterations_to",
-           "compare_starts", "save_weights", "load_weights"]
+           "warm_start", "compare_starts", "save_weights", "load_weights"]
 
 
 def dd_plan(npix, nang):
@@ -137,10 +137,13 @@ def train(net, inputs, targets, steps, batch=16, lr=1e-3, seed=0, crop=None, log
 def iterations_to(plan, b, x0, target, max_iters, pc_beta=0.0):
     """Plain SIRT from x0 until every slice's residual is <= target[slice] (residual of
     the iterate after k steps, evaluated after every step).  Returns (iterations per
-    slice, final iterate); a slice that never gets there counts max_iters + 1."""
+    slice, iterate per slice after exactly that many steps); a slice that never gets
+    there counts max_iters + 1 and returns its iterate after max_iters steps."""
     nz = b.shape[0]
     R, C = plan.sirt_weights()
     x = x0.clone().contiguous()
+    out = x0.clone().contiguous()
+    prev = torch.empty_like(x)
     work = torch.empty_like(b)
     stats = torch.zeros(3 * nz, dtype=torch.float64, device=b.device)
     bn = torch.linalg.vector_norm(b.double().reshape(nz, -1), dim=1)
@@ -149,14 +152,39 @@ def iterations_to(plan, b, x0, target, max_iters, pc_beta=0.0):
     r0 = torch.as_tensor(residual(plan, x, b), device=b.device)
     need[r0 <= tgt] = 0
     for k in range(1, max_iters + 2):
+        prev.copy_(x)                                   # x_{k-1}
         plan.sirt_iterate(x, b, R, C, work, pc_beta=pc_beta, stats=stats)
         # stats[3z] = ||b - A x_{k-1}||^2: the residual of the iterate entering step k
         rk = stats[0::3].sqrt() / bn
         newly = (rk <= tgt) & (need > max_iters)
         need[newly] = k - 1
+        out = torch.where(newly[:, None, None], prev, out)
         if k % 16 == 0 and bool((need <= max_iters).all()):
             break
-    return need.cpu().numpy(), x
+    # slices that never got there: prev holds x_{max_iters} after the last step
+    out = torch.where((need > max_iters)[:, None, None], prev, out)
+    return need.cpu().numpy(), out
+
+
+def warm_start(plan, net, b, x_in=None, pc_beta=0.0):
+    """x0 = net(x_in) (x_in defaults to FBP(b)) per slice, with plain SIRT as the
+    fallback: a slice whose network output is not finite, or whose residual
+    ||A x0 - b|| / ||b|| is not below that of the zero start (1) and of x_in, starts
+    from zero instead.  Non-negativity (pc_beta >= 0) is applied to x0 as SIRT applies
+    it to every iterate.  Returns (x0, mask of slices that fell back)."""
+    if x_in is None:
+        x_in = fbp(plan, b)
+    x0 = net(x_in).contiguous()
+    if pc_beta >= 0:
+        x0 = torch.where(x0 < 0, x0 * np.float32(pc_beta), x0)
+    finite = torch.isfinite(x0.reshape(x0.shape[0], -1)).all(dim=1)
+    x_safe = torch.where(finite[:, None, None], x0, torch.zeros_like(x0))
+    r_net = torch.as_tensor(residual(plan, x_safe, b), device=b.device)
+    r_in = torch.as_tensor(residual(plan, x_in.contiguous(), b), device=b.device)
+    ok = finite & (r_net < 1.0) & (r_net < r_in)
+    fell_back = ~ok
+    x0 = torch.where(ok[:, None, None], x_safe, torch.zeros_like(x_safe)).contiguous()
+    return x0, fell_back.cpu().numpy()
 
 
 def _time_ms(fn, reps=3):
@@ -194,9 +222,11 @@ def compare_starts(plan, net, b, plain_iters, max_iters, input_iters=0):
     t_net = _time_ms(lambda: net(x_in))
     t_in = t_fbp if input_iters == 0 else input_iters * t_iter
     kz, kf, kn = int(k_zero.max()), int(k_fbp.max()), int(k_net.max())
-    x_plain_k = sirt(plan, b, kz)
-    rel_to_plain = float(torch.linalg.vector_norm((x_w - x_plain_k).double())
-                         / torch.linalg.vector_norm(x_plain_k.double()))
+    # each slice's warm iterate at its own stopping step against plain's at its own
+    diff = torch.linalg.vector_norm((x_w - x_z).double().reshape(nz, -1), dim=1)
+    ref = torch.linalg.vector_norm(x_z.double().reshape(nz, -1), dim=1)
+    rel_slices = (diff / ref).cpu().numpy()
+    rel_to_plain = float(torch.linalg.vector_norm(diff) / torch.linalg.vector_norm(ref))
     return {
         "slices": nz, "npix": plan.npix, "nang": plan.nang, "plain_iters": plain_iters,
         "target_residual_max": float(target.max()),
@@ -207,6 +237,7 @@ def compare_starts(plan, net, b, plain_iters, max_iters, input_iters=0):
         "input_iters": input_iters,
         "seconds_unet": (t_in + t_net + kn * t_iter) / 1e3,
         "rel_l2_unet_vs_plain": rel_to_plain,
+        "rel_l2_unet_vs_plain_max_slice": float(rel_slices.max()),
         "residual_fbp": float(residual(plan, x_fbp, b).max()),
         "residual_unet": float(residual(plan, x_net, b).max()),
     }

@@ -45,9 +45,11 @@ def sirt(x0, b, A, At, n_iter, pc=-1.0, dtype=np.float32):
     return x, np.array(info), np.array(resid)
 
 
-def mlem(x0, b, A, At, n_iter, dtype=np.float32):
+def mlem(x0, b, A, At, n_iter, dtype=np.float32, eps=None):
     """Returns (x, err) with err[k] = sum (A x_k - b)^2 / sum b^2 (sir.py:287),
-    evaluated in float64 from the float32 arrays."""
+    evaluated in float64 from the float32 arrays.  With ``eps`` the loop is the
+    reference's `while it < n_iter and err > eps` (sir.py:251): it ends after the first
+    step whose err is <= eps."""
     n = x0.shape[-1]
     nang = b.shape[-2]
     d = At(np.ones((nang, n), dtype=dtype)).astype(dtype)
@@ -64,17 +66,20 @@ def mlem(x0, b, A, At, n_iter, dtype=np.float32):
         err.append(np.sum((bn.astype(np.float64) - b) ** 2) / np.sum(b.astype(np.float64) ** 2))
         xa = At(q).astype(dtype)
         x = (x * (inv_d * xa).astype(dtype)).astype(dtype)   # sir.py:283
+        if eps is not None and not err[-1] > eps:
+            break
     return x, np.array(err)
 
 
 def sps(x0, b, A, At, n_iter, dtype=np.float32, penalty=None, beta=0.0, delta=0.1,
-        clip=True):
+        clip=True, eps=None):
     """sir.py:348-560: w = exp(-b/max b); den = At(w*A(1));
     x <- x - (At(w*(A x - b)) + beta R') / (den + beta R'' + spacing(1)), the quotient
     evaluated in float64 (the GPU path's FP64 denominator), R from the restated penalty
     module (oracle/penalty_oracle.c, the reference build's arithmetic) on x, or on the
     scipy Sobel magnitude of x for "huber"; then np.clip(x, 0, max x) when ``clip``
-    (sir.py:539-540).  Returns (x, err) with err[k] = sum(w(Ax_k-b))^2/sum b^2."""
+    (sir.py:539-540).  Returns (x, err) with err[k] = sum(w(Ax_k-b))^2/sum b^2 (one
+    slice).  With ``eps`` the loop is sir.py:426's `while it < n_iter and err > eps`."""
     n = x0.shape[-1]
     b = b.astype(dtype)
     # the reference's float32 np.exp is not correctly rounded; like the GPU path, take
@@ -101,6 +106,8 @@ def sps(x0, b, A, At, n_iter, dtype=np.float32, penalty=None, beta=0.0, delta=0.
         x = (x.astype(np.float64) - p / q).astype(dtype)
         if clip:
             x = np.clip(x, 0, np.max(x))
+        if eps is not None and not err[-1] > eps:
+            break
     return x, np.array(err)
 
 

@@ -82,3 +82,32 @@ def test_graph_of_batch_equals_eager():
             x_ref, _ = eager_sirt(m.plan, m.b, 3 * rep)
             assert torch.equal(m.x, x_ref), "replay %d" % rep
     g.close()
+
+
+def test_graph_keeps_its_scratch_when_the_plan_buffer_grows():
+    # The graph replays into its own scratch: calls that make the plan grow its buffer
+    # (more slices) or lay it out for other launch shapes between replays must not
+    # change what a replay computes.
+    p = dd_plan(96, 60)
+    b = sinos(p, 1, 7)
+    R, C = p.sirt_weights()
+    x = torch.zeros((1, 96, 96), dtype=torch.float32, device="cuda")
+    w = torch.empty_like(b)
+    st = torch.zeros(3, dtype=torch.float64, device="cuda")
+    g = Graph.sirt(p, x, b, R, C, w, pc_beta=0.0, stats=st, iters=2)
+    g.launch()
+    big = sinos(p, 6, 11)                       # plan scratch grows for 6 slices
+    xb = torch.zeros((6, 96, 96), dtype=torch.float32, device="cuda")
+    wb = torch.empty_like(big)
+    p.sirt_iterate(xb, big, R, C, wb, pc_beta=0.0,
+                   stats=torch.zeros(18, dtype=torch.float64, device="cuda"))
+    fwd0, back0 = p.shapes
+    p.set_shapes((fwd0 + 1) % 3, (back0 + 1) % 3)   # another partial-sum layout
+    p.forward(xb)
+    g.launch()
+    torch.cuda.synchronize()
+    p.set_shapes(fwd0, back0)
+    x_ref, st_ref = eager_sirt(p, b, 4)
+    assert torch.equal(x, x_ref)
+    assert torch.equal(st, st_ref)
+    g.close()

@@ -158,3 +158,29 @@ def test_admm_inner_cg_stops_on_an_exactly_solved_slice(which):
     np.testing.assert_array_equal(x_ref, 0.0)
     x1_ref, _ = ref(np.ones((n, n), np.float32), b1, A, At, it, 4, 0.5, 0.2, 1.0)
     assert rel(x[1], x1_ref) <= 1e-4
+
+
+def test_dropin_em_and_sps_honour_eps():
+    # sir.py:256 / 426: the loop ends right after the first step whose err <= eps
+    n, v = 48, 40
+    deg = angles_deg(v)
+    tp = projectors(n, deg, oper="dd")
+    A = lambda z: O.dd_forward(z, tp.angles, O.FAST)
+    At = lambda z: O.dd_back(z, tp.angles, O.FAST)
+    b = A(ellipses(n, 6, seed=8)) + np.float32(0.25)
+    _, err_full = S.mlem(np.ones((n, n), np.float32), b, A, At, 20)
+    eps = 0.5 * float(err_full[6] + err_full[7])    # far from both (float rounding)
+    x_ref, err_ref = S.mlem(np.ones((n, n), np.float32), b, A, At, 20, eps=eps)
+    assert len(err_ref) < 20
+    xe, infoe = sir.em(np.ones((1, n, n), np.float32), b[None], tp,
+                       param(npix_op=n, nang=v, n_iter=20, eps=eps))
+    assert infoe.shape == (len(err_ref), 2)
+    np.testing.assert_array_equal(xe[0], x_ref)
+    _, err_s = S.sps(np.ones((n, n), np.float32), b, A, At, 20)
+    eps_s = 0.5 * float(err_s[5] + err_s[6])
+    xs_ref, errs_ref = S.sps(np.ones((n, n), np.float32), b, A, At, 20, eps=eps_s)
+    assert len(errs_ref) < 20
+    xs, infos = sir.sps(np.ones((1, n, n), np.float32), b[None], tp,
+                        param(npix_op=n, nang=v, n_iter=20, eps=eps_s))
+    assert infos.shape == (len(errs_ref), 2)
+    np.testing.assert_allclose(xs[0], xs_ref, rtol=0, atol=1e-5 * np.abs(xs_ref).max())

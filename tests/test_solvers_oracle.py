@@ -241,3 +241,29 @@ def test_library_taps_are_the_windows_impulse_response(ftype):
                                             h.ctypes.data_as(ctypes.c_void_p)) == 0
         ref = np.fft.ifft(S.calc_filter(nfreq, ftype)).real[:npix]
         np.testing.assert_allclose(h, ref, rtol=0, atol=1e-12)
+
+
+def test_mlem_and_sps_eps_stop_after_the_first_step_at_or_below_eps():
+    # sir.py:256 / 426: `while it < n_iter and err > eps`, err of the iterate entering the
+    # step (sir.py:287, 546).  The loop runs the step that finds err <= eps and then ends,
+    # so the stopped run is the unstopped one cut right after that step.
+    n, v = 16, 12
+    facade = (angles_deg(v) * np.pi / 180).astype(np.float32)
+    A = lambda z: O.dd_forward(z, facade, O.FAST)
+    At = lambda z: O.dd_back(z, facade, O.FAST)
+    img = ellipses(n, 4, seed=3)
+    b = A(img) + np.float32(0.5)
+    x_full, err_full = S.mlem(np.ones((n, n), np.float32), b, A, At, 12)
+    eps = float(err_full[5]) * (1 + 1e-9)    # err_full is decreasing here
+    assert np.all(np.diff(err_full[:7]) < 0)
+    x_eps, err_eps = S.mlem(np.ones((n, n), np.float32), b, A, At, 12, eps=eps)
+    assert len(err_eps) == 6
+    x_6, _ = S.mlem(np.ones((n, n), np.float32), b, A, At, 6)
+    np.testing.assert_array_equal(x_eps, x_6)
+    _, err_s = S.sps(np.zeros((n, n), np.float32), b, A, At, 12)
+    k = int(np.argmax(err_s <= err_s[4] * (1 + 1e-9)))
+    x_seps, err_seps = S.sps(np.zeros((n, n), np.float32), b, A, At, 12,
+                             eps=float(err_s[4]) * (1 + 1e-9))
+    assert len(err_seps) == k + 1
+    x_sk, _ = S.sps(np.zeros((n, n), np.float32), b, A, At, k + 1)
+    np.testing.assert_array_equal(x_seps, x_sk)

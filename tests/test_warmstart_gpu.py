@@ -40,3 +40,42 @@ def test_unet_warm_start_needs_fewer_iterations():
     assert r["iters_unet"] < r["iters_fbp"]
     assert r["iters_unet"] < r["iters_zero"]
     assert r["iters_zero"] <= 101
+
+
+def test_iterations_to_returns_the_iterate_of_the_counted_step():
+    # the iterate returned for a slice is plain SIRT after exactly the number of steps
+    # reported for it (so time and result of a start describe the same iterate)
+    plan = W.dd_plan(64, 60)
+    _, _, b, _ = W.make_pairs(plan, 3, 1, seed=4, noise=None, n_ellipses=6, texture=0)
+    x_plain = W.sirt(plan, b, 37)
+    # the stop test reads the residual from the fused step's FP64 monitor, summed in
+    # another order than residual(): a margin far above that rounding, far below a step
+    target = W.residual(plan, x_plain, b) * (1.0 + 1e-7)
+    zero = torch.zeros((3, 64, 64), device="cuda")
+    need, xs = W.iterations_to(plan, b, zero, target, max_iters=200)
+    assert (need <= 37).all() and (need >= 30).all()
+    for z in range(3):
+        ref = W.sirt(plan, b[z:z + 1].contiguous(), int(need[z]))
+        assert torch.equal(xs[z], ref[0]), (z, need[z])
+    # a target no slice reaches within max_iters: the iterate after max_iters steps
+    need2, xs2 = W.iterations_to(plan, b, zero, target * 1e-6, max_iters=20)
+    assert (need2 == 21).all()
+    assert torch.equal(xs2, W.sirt(plan, b, 20))
+
+
+def test_warm_start_falls_back_to_zero_on_a_bad_network_output():
+    plan = W.dd_plan(64, 60)
+    _, _, b, _ = W.make_pairs(plan, 3, 1, seed=5, noise=None, n_ellipses=6, texture=0)
+    good = W.sirt(plan, b, 30)
+
+    class Fake:
+        def __call__(self, x):
+            out = good.clone()
+            out[1] = float("nan")          # non-finite
+            out[2] = 50.0                  # far worse than its input
+            return out
+
+    x0, fell = W.warm_start(plan, Fake(), b, x_in=W.sirt(plan, b, 5))
+    assert fell.tolist() == [False, True, True]
+    assert torch.equal(x0[0], good[0])
+    assert not x0[1:].any()
