@@ -251,11 +251,13 @@ def cg(plan, x, b, n_iter):
 
 # ---- ADMM-TV --------------------------------------------------------------------------
 
-def admm_tv(plan, x, b, n_iter, n_inner=4, lambd1=1.0, lambd2=1.0, mu=1.0, pc=-1.0):
+def admm_tv(plan, x, b, n_iter, n_inner=4, lambd1=1.0, lambd2=1.0, mu=1.0, pc=-1.0,
+            return_state=False):
     """admm.py:470-646 (lasso_tv): per outer step an inner CG of n_inner iterations on
     (A^T A + lambd1 I + mu G^T G) x = A^T b + G^T(mu u - alpha) (cg.py:256-310), then
     u = soft(G x + alpha/mu, lambd2/mu), alpha += G x - u.  Returns (x, err) with
-    err[k] = ||G x - u|| over all slices (admm.py:597)."""
+    err[k] = ||G x - u|| over all slices (admm.py:597), and (x, err, u, alpha), each of
+    u and alpha (nz, 2, N, N), when ``return_state``."""
     x = _as3(x)
     b = _as3(b)
     nz, n = x.shape[0], x.shape[-1]
@@ -288,6 +290,8 @@ def admm_tv(plan, x, b, n_iter, n_inner=4, lambd1=1.0, lambd2=1.0, mu=1.0, pc=-1
                                          float(lambd2), _p(vw.scal), _p(vw.work), st),
                 "admm_tv_update")
         errs[it] = vw.scal.view(nz, 4)[:, 3].sum().sqrt()
+    if return_state:
+        return x, errs.cpu().numpy(), u, al
     return x, errs.cpu().numpy()
 
 
