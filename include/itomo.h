@@ -163,6 +163,21 @@ int itomo_sirt_iterate(const itomo_plan *plan, float *x, const float *b, const f
                        const float *C, float *work, int nslices, float pc_beta,
                        double *stats, void *stream);
 
+/* The SIRT step of itomo_sirt_iterate with no device allocation (no counterpart in the
+ * reference; for callers that cannot allocate: code under hipGraph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  Same arguments, results and
+ * kernels; the plan's scratch is replaced by a fixed device array of
+ * ITOMO_NOALLOC_SCRATCH_BYTES per GPU, held until the process ends.  Calls that need
+ * more (itomo_sirt_iterate needs about 4*npix*npix bytes per slice plus FP64 partial
+ * sums: up to 3 slices of 2048^2 x 1800) fail with ITOMO_EINVAL.  Every call on one GPU
+ * shares that array: calls that overlap on one GPU (other streams, other threads) race
+ * and must be serialized by the caller.  The first call on each device looks up the
+ * array's device address. */
+#define ITOMO_NOALLOC_SCRATCH_BYTES (64u << 20)
+int itomo_sirt_iterate_noalloc(const itomo_plan *plan, float *x, const float *b,
+                               const float *R, const float *C, float *work, int nslices,
+                               float pc_beta, double *stats, void *stream);
+
 /* ---------------------------------------------------------------------------------
  * MLEM (algorithms/sir.py:204-283):
  *   inv_d = 1 / d, d = A^T(1) with zeros replaced by 1e-10 (sir.py:215-217, 283).

@@ -47,6 +47,7 @@ _SIGS = {
     "itomo_sirt_forward": ([_vp, _vp, _vp, _vp, _vp, _i, _vp, _vp], _i),
     "itomo_sirt_back": ([_vp, _vp, _vp, _vp, _i, _f, _vp, _vp], _i),
     "itomo_sirt_iterate": ([_vp, _vp, _vp, _vp, _vp, _vp, _i, _f, _vp, _vp], _i),
+    "itomo_sirt_iterate_noalloc": ([_vp, _vp, _vp, _vp, _vp, _vp, _i, _f, _vp, _vp], _i),
     "itomo_mlem_weights": ([_vp, _vp, _vp], _i),
     "itomo_mlem_forward": ([_vp, _vp, _vp, _vp, _i, _vp, _vp], _i),
     "itomo_mlem_back": ([_vp, _vp, _vp, _vp, _i, _vp, _vp], _i),
@@ -97,6 +98,7 @@ BACK_VARIANTS = 3     # ITOMO_BACK_VARIANTS
 FILTERS = {"ramp": 0, "shepp-logan": 1, "cosine": 2, "hamming": 3, "hanning": 4, "none": 5}
 
 VEC_WORK_PER_SLICE = 771
+NOALLOC_SCRATCH_BYTES = 64 << 20   # ITOMO_NOALLOC_SCRATCH_BYTES
 
 EXPORTS = tuple(_SIGS)
 

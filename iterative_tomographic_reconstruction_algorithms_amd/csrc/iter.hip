@@ -1,6 +1,8 @@
 // Iteration drivers of libitomo: SIRT (art.py:211-237) and MLEM (sir.py:204-283) built
 // from the projector kernels with fused epilogues, plus fills, FP64 reductions and
 // the elementwise epilogue used after the ray-driven scatter back-projection.
+#include <mutex>
+
 #include "epilogue.h"
 
 namespace itomo {
@@ -371,6 +373,71 @@ int itomo_mlem_iterate(const itomo_plan* p, float* x, const float* b, const floa
     int rc = itomo_mlem_forward(p, x, b, work, nslices, stats, stream);
     if (rc) return rc;
     return itomo_mlem_back(p, work, inv_d, x, nslices, stats, stream);
+}
+
+}  // extern "C"
+
+// ---- SIRT step without allocation ---------------------------------------------------
+// The scratch of itomo_sirt_iterate_noalloc: a fixed device array, one copy per GPU,
+// held until the process ends.  Host code never passes `scratch` itself to a kernel or
+// a copy (on the host the name is a stand-in, not a device address); it uses the
+// address hipGetSymbolAddress returns for the current device.
+static __device__ __align__(256) unsigned char scratch[ITOMO_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_noalloc_mutex;
+void* g_noalloc_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device, looked up once per device.
+int noalloc_scratch(void** out)
+{
+    int dev = -1;
+    int rc = check_hip(hipGetDevice(&dev), "sirt_iterate_noalloc: hipGetDevice");
+    if (rc) return rc;
+    if (dev < 0 || dev >= kMaxDevices)
+        return set_error(ITOMO_EINVAL, "sirt_iterate_noalloc: device %d beyond %d", dev,
+                         kMaxDevices);
+    std::lock_guard<std::mutex> lock(g_noalloc_mutex);
+    if (!g_noalloc_ptr[dev]) {
+        void* ptr = nullptr;
+        rc = check_hip(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)),
+                       "sirt_iterate_noalloc: hipGetSymbolAddress");
+        if (rc) return rc;
+        if (!ptr) return set_error(ITOMO_EHIP, "sirt_iterate_noalloc: null scratch address");
+        g_noalloc_ptr[dev] = ptr;
+    }
+    *out = g_noalloc_ptr[dev];
+    return ITOMO_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int itomo_sirt_iterate_noalloc(const itomo_plan* p, float* x, const float* b, const float* R,
+                               const float* C, float* work, int nslices, float pc_beta,
+                               double* stats, void* stream)
+{
+    int rc = check_iter(p, nslices, "sirt_iterate_noalloc");
+    if (rc) return rc;
+    if (!x || !b || !R || !C || !work)
+        return set_error(ITOMO_EINVAL, "sirt_iterate_noalloc: null array");
+    const size_t need = scratch_total(*p, nslices);
+    if (need > (size_t)ITOMO_NOALLOC_SCRATCH_BYTES)
+        return set_error(ITOMO_EINVAL, "sirt_iterate_noalloc: %d slice(s) of %dx%d with %d "
+                                       "views need %zu bytes of scratch, more than the %u "
+                                       "this entry point holds",
+                         nslices, p->npix, p->npix, p->nang, need,
+                         (unsigned)ITOMO_NOALLOC_SCRATCH_BYTES);
+    void* base = nullptr;
+    rc = noalloc_scratch(&base);
+    if (rc) return rc;
+    // Every scratch word a step reads is written earlier in the same step (transposed
+    // image, one partial-sum slot per block), so nothing needs clearing between calls.
+    ScratchScope scope(ScratchRegion{base, (size_t)ITOMO_NOALLOC_SCRATCH_BYTES});
+    return itomo_sirt_iterate(p, x, b, R, C, work, nslices, pc_beta, stats, stream);
 }
 
 int itomo_dot_f64(const float* a, const float* b, int64_t n, double* out, double* work,
