@@ -342,6 +342,51 @@ int itomo_filter_rows(const float *in, float *out, int nrows, int npix, const do
                       void *stream);
 
 /* ---------------------------------------------------------------------------------
+ * Gridding (gridrec) projector pair: the reference's Fourier-regridding back
+ * projection / filtered back projection (gridrec_v4_backproj.c:64 `gridrec_v4_backproj`,
+ * bound at gridrec_v4.pyx:7 and wrapped by gridrec_v4.pyx:32-62 `backproj`) and forward
+ * projection (gridrec_v4_forwproj.c:59 `gridrec_v4_forwproj`, gridrec_v4.pyx:67-94
+ * `forwproj`), used through class_projectors_grid.py:124-151 (A, At, fbp).
+ * itomo_gridrec_create takes the set-up the reference computes in Python
+ * (gridrec_lut.py:268 configure_regridding, class_projectors_grid.py:79-117) and in
+ * filters.c:151 calc_filter, all HOST arrays:
+ *   trig     : nang x {sin, cos} of the angles in degrees (backproj.c:308-309);
+ *   lut      : the kernel look-up table (nlut floats), ltbl = nlut - 5 its resolution
+ *              (class_projectors_grid.py:107);
+ *   deapod   : the 1-D deapodiser with its centring signs (pdim floats); the 2-D matrix
+ *              of the reference is deapod[j] * deapod[k];
+ *   lconv_back, lconv_fwd : kernel window widths of the back projection (param[6]) and
+ *              of the forward projection (fixed to 2*7/pi at forwproj.c:164);
+ *   interp   : 0 nearest, 1 linear LUT interpolation; radon_degree: 1 for the
+ *              first-derivative transform (zero edge padding, derivative filters);
+ *   filt_at, filt_fbp : pdim/2 complex (re, im) filter values of the plain and the
+ *              filtered back projection (calc_filter with type 0 and the chosen type);
+ *   phase    : pdim/2 complex centre-of-rotation phase of the forward projection
+ *              (forwproj.c:183-199, 363-364).
+ * pdim must be a product of 2, 3 and 5, at most 4096 (npix <= 2048 at the reference's
+ * oversampling 2 and 1.5).  Creating allocates the tables and two pdim x pdim complex
+ * grids plus nang x pdim complex rows of scratch; the projections allocate nothing and
+ * process `nslices` stacked slices one after another on `stream` (the object's scratch
+ * is shared: calls on one object must not overlap).  Angles are DEGREES, images
+ * npix x npix, sinograms nang x npix, row v = angle v.
+ * itomo_fft_rows: in-place FFT of nrows rows of L complex floats (interleaved re, im),
+ *   sign -1 = exp(-2 pi i jk/L) (FFTW_FORWARD), +1 = FFTW_BACKWARD, unnormalised;
+ *   L a product of 2, 3 and 5 up to 4096.  Synchronous (allocates a twiddle table).
+ * ------------------------------------------------------------------------------- */
+typedef struct itomo_gridrec itomo_gridrec;
+int itomo_gridrec_create(itomo_gridrec **g, int npix, int nang, int pdim,
+                         const float *trig_host, const float *lut_host, int nlut, int ltbl,
+                         const float *deapod_host, float lconv_back, float lconv_fwd,
+                         int interp, int radon_degree, const float *filt_at_host,
+                         const float *filt_fbp_host, const float *phase_host);
+int itomo_gridrec_destroy(itomo_gridrec *g);
+int itomo_gridrec_back(itomo_gridrec *g, const float *sino, float *image, int nslices,
+                       int filtered, void *stream);
+int itomo_gridrec_forward(itomo_gridrec *g, const float *image, float *sino, int nslices,
+                          void *stream);
+int itomo_fft_rows(void *data, int L, int nrows, int sign, void *stream);
+
+/* ---------------------------------------------------------------------------------
  * FP64 reductions used by the monitoring / CG / ADMM / SPS scalars.
  * out (DEVICE double[1]) = sum_i a[i]*b[i] over n float32 entries, accumulated in FP64
  * in a fixed order.  b may equal a.  work: DEVICE scratch of ITOMO_DOT_WORK doubles.

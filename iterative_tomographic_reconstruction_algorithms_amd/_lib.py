@@ -89,6 +89,12 @@ _SIGS = {
     "itomo_graph_launch": ([_vp, _vp], _i),
     "itomo_graph_destroy": ([_vp], _i),
     "itomo_filter_taps": ([_i, _i, _vp], _i),
+    "itomo_gridrec_create": ([ctypes.POINTER(_vp), _i, _i, _i, _vp, _vp, _i, _i, _vp, _f, _f, _i,
+                              _i, _vp, _vp, _vp], _i),
+    "itomo_gridrec_destroy": ([_vp], _i),
+    "itomo_gridrec_back": ([_vp, _vp, _vp, _i, _i, _vp], _i),
+    "itomo_gridrec_forward": ([_vp, _vp, _vp, _i, _vp], _i),
+    "itomo_fft_rows": ([_vp, _i, _i, _i, _vp], _i),
     "itomo_filter_rows": ([_vp, _vp, _i, _i, _vp, _vp], _i),
 }
 
