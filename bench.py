@@ -52,6 +52,32 @@ def measured_traffic(kernel):
     return None if v is None else float(v)
 
 
+def cpu_model():
+    """The host CPU's model name and logical core count (Linux /proc/cpuinfo)."""
+    name = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    name = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return name, os.cpu_count()
+
+
+def measured_limiter(kernel_key):
+    """Shader counters of the dominant kernels from the committed rocprofv3 summary
+    (profiles/counters_dd_sirt_2048x1800.json), or None."""
+    path = os.path.join(ROOT, "profiles", "counters_dd_sirt_2048x1800.json")
+    try:
+        with open(path) as f:
+            table = json.load(f)
+    except (OSError, ValueError):
+        return None
+    return table.get("kernels", {}).get(kernel_key)
+
+
 def cpu_baseline(img, c_angles, budget_views=20):
     """Reference C (radon_dd.c built with its own flags into oracle/_ref) timed on a
     bounded sample: forward + back of `budget_views` evenly spaced views at full N,
@@ -76,10 +102,36 @@ def cpu_baseline(img, c_angles, budget_views=20):
     t2 = time.perf_counter()
     secs = t2 - t0
     gups = 2.0 * img.shape[0] ** 2 * len(sub) / secs / 1e9
+    model, ncpu = cpu_model()
     return {"value": round(gups, 6), "unit": "Gupdates/s", "cores": 1, "kind": kind,
+            "cpu_model": model, "host_logical_cpus": ncpu,
             "sample": "forward+back of %d of the %d views at %dx%d (%.1f s fwd, %.1f s back), "
                       "single thread" % (len(sub), len(c_angles), img.shape[0], img.shape[0],
                                           t1 - t0, t2 - t1)}
+
+
+def sirt_to_residual(plan, x, b, work, stats, bnorm, target, max_iters, check_every=5):
+    """Plain SIRT from x (non-negativity on) until ||A x_k - b|| / ||b|| <= target, the
+    norms in FP64 on the GPU.  Each step's forward half computes the residual of the
+    iterate entering it; every `check_every` steps the host reads it before the back half,
+    and the run stops there with x = x_k, so `iterations` is exactly k.  The R/C setup is
+    inside the timed region.  Returns the timing record."""
+    import torch
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    R, C = plan.sirt_weights()
+    it, rel = 0, float("inf")
+    while True:
+        plan.sirt_forward(x, b, R, work, stats=stats)     # residual of x_it
+        if it % check_every == 0 or it == max_iters:
+            rel = float(stats[0].sqrt()) / bnorm
+            if rel <= target or it == max_iters:
+                break
+        plan.sirt_back(work, C, x, pc_beta=0.0, stats=stats)
+        it += 1
+    torch.cuda.synchronize()
+    return {"target": target, "iterations": it, "seconds": round(time.perf_counter() - t0, 4),
+            "reached": rel <= target, "residual": rel, "checked_every": check_every}
 
 
 def _events_ms(fn, reps):
@@ -328,6 +380,14 @@ def main():
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": measured_traffic(dom),
+                     # achieved counts algorithmic bytes: one 4-byte operand gather per
+                     # pixel-view plus the compulsory streams; the gathers are served by
+                     # LDS and L2, so HBM moves far less (traffic) and the kernels are
+                     # limited by VALU issue and LDS (measured below)
+                     "accounting": "algorithmic bytes: 4 B gathered per pixel-view + compulsory "
+                                   "input/output streams, per launch",
+                     "limiter": {"forward": measured_limiter("forward"),
+                                 "back": measured_limiter("back")},
                      "pair": {"what": "forward + back of one SIRT step",
                               "alg_bytes": alg_bytes_forward(N, V) + alg_bytes_back(N, V),
                               "achieved": round(pair_gbs, 1),
@@ -348,21 +408,8 @@ def main():
     if args.max_residual_iters > 0:
         x.zero_()
         torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        R, C = plan.sirt_weights()
-        it, rel = 0, 1.0
-        while it < args.max_residual_iters:
-            plan.sirt_iterate(x, b, R, C, work, pc_beta=0.0, stats=stats)
-            it += 1
-            if it % 5 == 0:
-                rel = float(stats[0].sqrt()) / bnorm  # residual of the iterate entering step
-                if rel <= args.residual_target:
-                    break
-        torch.cuda.synchronize()
-        line["plain_sirt_to_residual"] = {"target": args.residual_target, "iterations": it,
-                                          "seconds": round(time.perf_counter() - t0, 4),
-                                          "reached": rel <= args.residual_target,
-                                          "residual": rel}
+        line["plain_sirt_to_residual"] = sirt_to_residual(
+            plan, x, b, work, stats, bnorm, args.residual_target, args.max_residual_iters)
 
     if args.extra_configs:
         line["extra"] = extra_configs(cpu=not args.no_cpu_baseline)

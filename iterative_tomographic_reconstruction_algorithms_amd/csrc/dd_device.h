@@ -214,7 +214,15 @@ __device__ __forceinline__ void dd_back_tile(const DdView* __restrict__ geom,
     __shared__ float sLs[kBackViews][T];   // L* of the tile's lines per view
 
     const int tid = threadIdx.x;
-    const int tx = tid % TP, ty = tid / TP;
+    // Threads -> micro-tiles: each wave takes an 8 x 8 block of micro-tiles (2 x 2 waves
+    // per 16 x 16 block), so the 32 lanes of a half-wave cover a 32 x 16-pixel patch
+    // (P = 4) whose projection onto the detector spans at most 28|c| + 12|s| < 31 bins:
+    // the staged bins they read in one step lie in distinct LDS banks.  (Rows of 16
+    // micro-tiles per half-wave span up to 60 bins and conflicted.)
+    static_assert(TP == 8 || TP == 16, "micro-tile grid");
+    const int lane = tid & 63, wv = tid >> 6;
+    const int tx = TP == 16 ? ((wv & 1) << 3) + (lane & 7) : tid % TP;
+    const int ty = TP == 16 ? ((wv >> 1) << 3) + (lane >> 3) : tid / TP;
     const int r0 = by * T, q0 = bx * T;
     const int rr = r0 + P * ty, cc = q0 + P * tx;  // micro-tile origin
     sino += (int64_t)z * sino_stride;
