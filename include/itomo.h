@@ -178,6 +178,43 @@ int itomo_sirt_iterate_noalloc(const itomo_plan *plan, float *x, const float *b,
                                const float *R, const float *C, float *work, int nslices,
                                float pc_beta, double *stats, void *stream);
 
+/* SIRT constraints after a step (art.py:296-321), on the reconstruction square
+ * [i1, i2) x [i1, i2) of each of nslices npix x npix images, in the reference's order:
+ *   mask (m x m uint8, m = i2 - i1, or NULL): x = 0 where mask == 0 (utils.supp_constr,
+ *     utils.py:264-266);
+ *   mask_add (nmask x m x m uint8): for each mask in turn, x = mean of x over mask == 1
+ *     there (utils.mask_constr, utils.py:279-281; the mean is summed in FP64 and rounded
+ *     to float, where numpy sums the float32 values pairwise);
+ *   pc == 1: x < 0 -> 0, then zero outside the resolution circle (centre m/2 - 1/2,
+ *     radius m/2 - 1/2; utils.resol_circle_constr, utils.py:229-251); pc == 2: the circle
+ *     only; other pc >= 0: x[x < 0] *= pc (utils.phys_constr); pc < 0: nothing.
+ * x_old and stats (DEVICE double[3*nslices]) optional: stats[1] = sum (x - x_old)^2 and
+ * stats[2] = sum x_old^2 over the whole image after the constraints (the monitor of
+ * art.py:325).  work: DEVICE double scratch of ITOMO_CONSTR_WORK(nslices, npix) entries,
+ * needed with masks or stats. */
+#define ITOMO_CONSTR_WORK(nslices, npix) \
+    ((int64_t)(nslices) * 3 * (((int64_t)(npix) * (npix) + 2047) / 2048))
+int itomo_sirt_constraints(float *x, int npix, int nslices, int i1, int i2,
+                           const unsigned char *mask, const unsigned char *mask_add, int nmask,
+                           float pc, const float *x_old, double *stats, double *work,
+                           void *stream);
+
+/* The two halves of a SIRT step as elementwise passes, for projector pairs without
+ * fused epilogues (the gridding pair): with ax = A x and atw = A^T work computed by the
+ * caller, over nslices slices of n entries (R, C shared by all slices):
+ *   itomo_sirt_residual: ax_io <- R * (b - ax_io), stats[0] = sum (b - ax)^2 (art.py:212);
+ *   itomo_sirt_update:   x <- x + C * atw, then x[x < 0] *= pc_beta when pc_beta >= 0,
+ *                        stats[1] = sum (x_new - x)^2, stats[2] = sum x^2;
+ *   itomo_reciprocal:    out = 1 / v with zeros replaced by 1e-10 first (art.py:227-237).
+ * stats (DEVICE double[3*nslices]) or NULL; work: DEVICE double scratch of
+ * ITOMO_EPI_WORK(nslices, n) entries when stats is given. */
+#define ITOMO_EPI_WORK(nslices, n) ((int64_t)(nslices) * 3 * (((int64_t)(n) + 2047) / 2048))
+int itomo_sirt_residual(float *ax_io, const float *b, const float *R, int64_t n, int nslices,
+                        double *stats, double *work, void *stream);
+int itomo_sirt_update(const float *atw, const float *C, float *x, int64_t n, int nslices,
+                      float pc_beta, double *stats, double *work, void *stream);
+int itomo_reciprocal(const float *v, float *out, int64_t n, void *stream);
+
 /* ---------------------------------------------------------------------------------
  * MLEM (algorithms/sir.py:204-283):
  *   inv_d = 1 / d, d = A^T(1) with zeros replaced by 1e-10 (sir.py:215-217, 283).

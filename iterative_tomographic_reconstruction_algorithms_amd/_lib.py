@@ -48,6 +48,10 @@ _SIGS = {
     "itomo_sirt_back": ([_vp, _vp, _vp, _vp, _i, _f, _vp, _vp], _i),
     "itomo_sirt_iterate": ([_vp, _vp, _vp, _vp, _vp, _vp, _i, _f, _vp, _vp], _i),
     "itomo_sirt_iterate_noalloc": ([_vp, _vp, _vp, _vp, _vp, _vp, _i, _f, _vp, _vp], _i),
+    "itomo_sirt_constraints": ([_vp, _i, _i, _i, _i, _vp, _vp, _i, _f, _vp, _vp, _vp, _vp], _i),
+    "itomo_sirt_residual": ([_vp, _vp, _vp, _i64, _i, _vp, _vp, _vp], _i),
+    "itomo_sirt_update": ([_vp, _vp, _vp, _i64, _i, _f, _vp, _vp, _vp], _i),
+    "itomo_reciprocal": ([_vp, _vp, _i64, _vp], _i),
     "itomo_mlem_weights": ([_vp, _vp, _vp], _i),
     "itomo_mlem_forward": ([_vp, _vp, _vp, _vp, _i, _vp, _vp], _i),
     "itomo_mlem_back": ([_vp, _vp, _vp, _vp, _i, _vp, _vp], _i),

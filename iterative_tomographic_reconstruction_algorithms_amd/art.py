@@ -1,15 +1,28 @@
-"""Drop-in for the SIRT functions of the reference's algorithms/art.py (art.py:211-435),
-running on the GPU.  Arrays may be numpy (returned as numpy) or float32 GPU tensors.
+"""Drop-in for the reference's algorithms/art.py (SIRT, art.py:211-435, and its command
+line, art.py:85-200, 588-860), running on the GPU.  Arrays may be numpy (returned as
+numpy) or float32 GPU tensors.
 
-Supported options: n_iter, pc (phys_constr, art.py:314-321).  TV, masks and the
-resolution-circle constraint are outside this path and raise NotImplementedError.
+Supported options: n_iter, eps, pc (phys_constr and the resolution circle,
+art.py:314-321), mask and mask_add (art.py:303-313), edge padding (the reconstruction
+square index_start:index_end), the projectors dist-driv, ray-driv, pix-driv, slant,
+grid-pswf and grid-kb, -init (start from the FBP, art.py:410-418), -cit (every iterate
+written, art.py:328-341).  TV denoising (-tv, an skimage routine) and the B-spline
+projector are outside this path and raise an error.
+
+    python -m iterative_tomographic_reconstruction_algorithms_amd.art -Di data/ -i sino.DMP \\
+        -pr dist-driv -n 100 -pc 0
 """
+import argparse
+import sys
+import time
+
 import numpy as np
 import torch
 
+from . import cli
 from . import solvers
 from .class_projectors_radon import projectors
-from .params import PROJECTOR_OPER
+from .params import PROJECTOR_OPER, param as make_param
 from .solvers import _dev
 
 myfloat = np.float32
@@ -35,38 +48,114 @@ def sirt_step(x, b, tp, R, C):
     return x
 
 
-def _pc_beta(param):
-    pc = getattr(param, "pc", -1.0)
-    if pc is None or pc < 0.0:
-        return -1.0
-    if pc in (1.0, 2.0):
-        raise NotImplementedError("pc=%g needs the resolution-circle constraint "
-                                  "(utils.resol_circle_constr), outside the GPU path" % pc)
-    return float(pc)
+def _region(param, n):
+    if getattr(param, "edge_padding", False):
+        i1 = int(param.index_start)
+        i2 = int(param.index_end) if param.index_end is not None else n
+        return (i1, i2)
+    return None
 
 
-def sirt(x, b, tp, param):
-    """art.py:217-365: returns (x, info) with info rows [diff, obj]."""
-    for opt in ("mask", "mask_add"):
-        if getattr(param, opt, None) is not None:
-            raise NotImplementedError("SIRT option %r is outside the GPU path" % opt)
+def sirt(x, b, tp, param, on_iterate=None, start=0):
+    """art.py:217-365: returns (x, info) with info rows [diff, obj].  The constraints
+    follow each step in the reference's order (support mask, additional masks, pc)."""
     if getattr(param, "tv", 0.0):
-        raise NotImplementedError("SIRT with TV denoising is outside the GPU path")
+        raise NotImplementedError("SIRT with TV denoising (an skimage routine) is outside "
+                                  "the GPU path")
     n_it = 200 if param.n_iter is None else int(param.n_iter)
     xd = _dev(x).clone()
     bd = _dev(b)
     plan = tp.plan(xd.shape[-1])
-    xd, info = solvers.sirt(plan, xd, bd, n_it, pc=_pc_beta(param), check_every=10)
+    mask_add = getattr(param, "mask_add", None)
+    if mask_add is not None:
+        mask_add = np.stack([np.asarray(m) for m in mask_add])
+    pc = getattr(param, "pc", -1.0)
+    xd, info = solvers.sirt(plan, xd, bd, n_it, pc=-1.0 if pc is None else float(pc),
+                            check_every=10, mask=getattr(param, "mask", None),
+                            mask_add=mask_add, region=_region(param, xd.shape[-1]),
+                            start=start, on_iterate=on_iterate)
     xd = xd.reshape(np.shape(x)) if not isinstance(x, torch.Tensor) else xd.reshape(x.shape)
     return _out(xd, x), info
 
 
-def alg_rec(b, a, param):
-    """art.py:378-435 for the radon projectors: x starts from ones (art.py:408)."""
-    oper = PROJECTOR_OPER.get(param.projector, param.projector)
+def alg_rec(b, a, param, x0=None, on_iterate=None, start=0):
+    """art.py:378-435: the projector of param.projector, x from ones (art.py:408) or from
+    the FBP (-init, art.py:410-418) or from x0 (a resumed run), then SIRT."""
     n = param.npix_op
-    tp = projectors(n, a, oper=oper)
+    tp = cli.make_projector(param.projector, n, a, getattr(param, "radon_degree", 0))
     b = np.asarray(b, dtype=myfloat) if not isinstance(b, torch.Tensor) else b
     nz = b.shape[0] if b.ndim == 3 else 1
-    x = np.ones((nz, n, n), dtype=myfloat)
-    return sirt(x, b.reshape(nz, -1, n) if not isinstance(b, torch.Tensor) else b, tp, param)
+    b = b.reshape(nz, -1, n)
+    if x0 is not None:
+        x = np.asarray(x0, dtype=myfloat).reshape(nz, n, n)
+    else:
+        x = np.ones((nz, n, n), dtype=myfloat)
+        if getattr(param, "init_object", False):
+            for i in range(nz):
+                x[i] = np.asarray(tp.fbp(b[i]))
+    return sirt(x, b, tp, param, on_iterate=on_iterate, start=start)
+
+
+# ---- command line (art.py:85-200, 588-860) ---------------------------------------------
+
+def getArgs(argv=None):
+    parser = argparse.ArgumentParser(description="Algebraic recontruction algorithms",
+                                     formatter_class=argparse.ArgumentDefaultsHelpFormatter)
+    cli.add_common_args(parser, "grid-kb")
+    parser.add_argument("-o", "--reco", dest="reco",
+                        help="Specify name of output reconstruction")
+    parser.add_argument("-n", dest="n_iter", type=int, help="Specify number of iterations")
+    parser.add_argument("-a", dest="algorithm", default="sirt", help="Specify algorithm: sirt")
+    parser.add_argument("-pc", dest="pc", type=myfloat, default=-1.0,
+                        help="-1 off; in [0, 1): x<0 *= pc; 1: x<0 -> 0 and the resolution "
+                             "circle; 2: the resolution circle")
+    parser.add_argument("-tv", dest="tv", type=myfloat, default=0.0, help="Total variation")
+    parser.add_argument("-mask", dest="mask", help="Input object support")
+    parser.add_argument("-mask_add", dest="mask_add",
+                        help="Input additional masks for a priori knowledge (comma list)")
+    parser.add_argument("-dpc", dest="dpc", action="store_true",
+                        help="Enable DPC reconstruction")
+    parser.add_argument("-dbp", dest="dbp", action="store_true",
+                        help="Enable DBP reconstruction")
+    args = parser.parse_args(argv)
+    if args.n_iter is None and args.eps is None:
+        parser.print_help()
+        sys.exit("\nERROR: Stopping criterion not specified!\n")
+    return args
+
+
+def main(argv=None):
+    args = getArgs(argv)
+    if args.dbp:
+        raise SystemExit("\nERROR: -dbp (Savitzky-Golay differential sinogram) is not "
+                         "available here\n")
+    d = cli.prepare(args)
+    nz = len(d["names"])
+    # class_art_param.py:14-138
+    p = make_param(npix_op=d["npix"], nang=d["nang"], nz=nz, n_iter=args.n_iter,
+                   eps=args.eps, pc=float(args.pc), tv=float(args.tv),
+                   projector=args.projector, algorithm=args.algorithm,
+                   init_object=args.init_object, radon_degree=1 if args.dpc else 0,
+                   edge_padding=bool(d["edf"]), index_start=d["i1"], index_end=d["i2"])
+    p.mask = None if args.mask is None else cli.read_image(args.mask).astype(np.uint8)
+    p.mask_add = None if args.mask_add is None else [
+        cli.read_image(f).astype(np.uint8) for f in args.mask_add.split(",")]
+    root = "_sirt" + cli.projector_suffix(args.projector)
+    writer = cli.checkpoint_writer(args, root)
+    on_it = None if writer is None else (lambda k, x: writer(k, x, d["i1"], d["i2"]))
+    x0 = None if args.resume is None else cli.load_resume(args, nz, d["npix"])
+    t0 = time.time()
+    reco, info = alg_rec(d["sinos"], d["angles"], p, x0=x0, on_iterate=on_it,
+                         start=args.it0)
+    t1 = time.time()
+    outs = []
+    for i in range(nz):
+        r = reco[i, d["i1"]:d["i2"], d["i1"]:d["i2"]] if d["edf"] else reco[i]
+        outs.append(cli.save_reco(d["pathout"], d["names"][i],
+                                  args.reco if nz == 1 else None, root, r))
+    print("\nTime elapsed for the reconstruction: ", (t1 - t0) / 60.0)
+    return outs, info
+
+
+if __name__ == "__main__":
+    main()

@@ -40,6 +40,13 @@ class projectors:
         self.param_filt = np.array([ctr, p.filter_type, 0, oversampl, interp1, len(p.lut) - 5,
                                     p.W, radon_degree]).astype(myfloat)
 
+    def plan(self, npix=None):
+        """The GPU geometry (gridrec.GridrecPlan); the solvers drive it like core.Plan."""
+        if npix is not None and int(npix) != self._plan.npix:
+            raise ValueError("this projector is set up for %d pixels, not %d"
+                             % (self._plan.npix, npix))
+        return self._plan
+
     @staticmethod
     def _in(x):
         if isinstance(x, torch.Tensor):

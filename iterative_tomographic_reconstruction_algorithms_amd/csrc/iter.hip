@@ -440,6 +440,47 @@ int itomo_sirt_iterate_noalloc(const itomo_plan* p, float* x, const float* b, co
     return itomo_sirt_iterate(p, x, b, R, C, work, nslices, pc_beta, stats, stream);
 }
 
+int itomo_sirt_residual(float* ax_io, const float* b, const float* R, int64_t n, int nslices,
+                        double* stats, double* work, void* stream)
+{
+    if (!ax_io || !b || !R || n < 1 || nslices < 1 || nslices > 65535 || (stats && !work))
+        return set_error(ITOMO_EINVAL, "sirt_residual: bad argument");
+    hipStream_t st = (hipStream_t)stream;
+    EpiArgs e{};
+    e.b = b;
+    e.w = R;
+    e.w_stride = 0;
+    e.partials = stats ? work : nullptr;
+    int bps = 0;
+    int rc = elementwise_fwd_epilogue(EPI_SIRT_FWD, ax_io, e, n, nslices, st, &bps);
+    if (rc || !stats) return rc;
+    return finalize_partials(work, bps, nslices, 0, 1, stats, st);
+}
+
+int itomo_sirt_update(const float* atw, const float* C, float* x, int64_t n, int nslices,
+                      float pc_beta, double* stats, double* work, void* stream)
+{
+    if (!atw || !C || !x || n < 1 || nslices < 1 || nslices > 65535 || (stats && !work))
+        return set_error(ITOMO_EINVAL, "sirt_update: bad argument");
+    hipStream_t st = (hipStream_t)stream;
+    EpiArgs e{};
+    e.w = C;
+    e.w_stride = 0;
+    e.pc_beta = pc_beta;
+    e.partials = stats ? work : nullptr;
+    int rc = elementwise_back_epilogue(EPI_SIRT_BACK, atw, x, e, n, nslices, st);
+    if (rc || !stats) return rc;
+    return finalize_partials(work, (int)((n + kEpiChunk - 1) / kEpiChunk), nslices, 1, 3, stats,
+                             st);
+}
+
+int itomo_reciprocal(const float* v, float* out, int64_t n, void* stream)
+{
+    if (!v || !out || n < 1) return set_error(ITOMO_EINVAL, "reciprocal: bad argument");
+    EpiArgs e{};
+    return elementwise_back_epilogue(EPI_RECIP, v, out, e, n, 1, (hipStream_t)stream);
+}
+
 int itomo_dot_f64(const float* a, const float* b, int64_t n, double* out, double* work,
                   void* stream)
 {

@@ -247,3 +247,64 @@ class GridrecPlan:
                                            ctypes.c_void_p(out.data_ptr()), nz, int(filtered),
                                            stream_handle()), "gridrec_back")
         return out
+
+    # -- SIRT on the gridding pair (art.py:211-237 with the grid-pswf / grid-kb projectors):
+    # the projections, then the elementwise halves of the step (itomo_sirt_residual /
+    # itomo_sirt_update), with the method signatures of core.Plan so that solvers.sirt
+    # drives either.
+    @property
+    def image_numel(self):
+        return self.npix * self.npix
+
+    @property
+    def sino_numel(self):
+        return self.nang * self.npix
+
+    def _work(self, nz, n):
+        need = max(1, 3 * nz * ((n + 2047) // 2048))
+        w = getattr(self, "_epi_work", None)
+        if w is None or w.numel() < need:
+            w = torch.empty(need, dtype=torch.float64, device=device())
+            self._epi_work = w
+        return w
+
+    def sirt_weights(self):
+        """(R, C) = (1/A(1), 1/A^T(1)) with zero sums replaced by 1e-10 (art.py:226-237)."""
+        dev = device()
+        ones_img = torch.ones((self.npix, self.npix), dtype=torch.float32, device=dev)
+        ones_sino = torch.ones((self.nang, self.npix), dtype=torch.float32, device=dev)
+        R = self.forward(ones_img)
+        C = self.back(ones_sino)
+        st = stream_handle()
+        L.check(L.lib().itomo_reciprocal(ctypes.c_void_p(R.data_ptr()),
+                                         ctypes.c_void_p(R.data_ptr()), R.numel(), st),
+                "reciprocal")
+        L.check(L.lib().itomo_reciprocal(ctypes.c_void_p(C.data_ptr()),
+                                         ctypes.c_void_p(C.data_ptr()), C.numel(), st),
+                "reciprocal")
+        return R, C
+
+    def sirt_forward(self, x, b, R, work, stats=None):
+        """work = R * (b - A x); stats[3z] = sum (b - A x)^2."""
+        nz = x.numel() // self.image_numel
+        work.copy_(self.forward(x.contiguous()).reshape(work.shape))
+        p = lambda t: ctypes.c_void_p(t.data_ptr()) if t is not None else None
+        L.check(L.lib().itomo_sirt_residual(p(work), p(b), p(R), self.sino_numel, nz, p(stats),
+                                            p(self._work(nz, self.sino_numel)),
+                                            stream_handle()), "sirt_residual")
+        return work
+
+    def sirt_back(self, work, C, x, pc_beta=-1.0, stats=None):
+        """x = x + C * A^T work, then x[x < 0] *= pc_beta when pc_beta >= 0."""
+        nz = x.numel() // self.image_numel
+        atw = self.back(work)
+        p = lambda t: ctypes.c_void_p(t.data_ptr()) if t is not None else None
+        L.check(L.lib().itomo_sirt_update(p(atw), p(C), p(x), self.image_numel, nz,
+                                          float(pc_beta), p(stats),
+                                          p(self._work(nz, self.image_numel)), stream_handle()),
+                "sirt_update")
+        return x
+
+    def sirt_iterate(self, x, b, R, C, work, pc_beta=-1.0, stats=None):
+        self.sirt_forward(x, b, R, work, stats)
+        return self.sirt_back(work, C, x, pc_beta, stats)

@@ -41,13 +41,64 @@ class VecWork:
 
 # ---- SIRT -----------------------------------------------------------------------------
 
-def sirt(plan, x, b, n_iter, pc=-1.0, check_every=0):
+def _mask_u8(m, cond, side, dev):
+    a = np.asarray(m.cpu().numpy() if isinstance(m, torch.Tensor) else m)
+    if a.shape[-2:] != (side, side):
+        raise ValueError("mask of shape %s for a %dx%d reconstruction region"
+                         % (a.shape, side, side))
+    return torch.from_numpy(np.ascontiguousarray(cond(a).astype(np.uint8))).to(dev)
+
+
+class Constraints:
+    """The SIRT constraints of art.py:296-321 applied after each step on the GPU
+    (itomo_sirt_constraints): support mask, additional masks, pc (non-negativity,
+    resolution circle), on the region [i1, i2)^2."""
+
+    def __init__(self, npix, nslices, pc=-1.0, mask=None, mask_add=None, region=None):
+        dev = device()
+        self.npix, self.nslices = npix, nslices
+        self.i1, self.i2 = (0, npix) if region is None else (int(region[0]), int(region[1]))
+        side = self.i2 - self.i1
+        self.pc = float(-1.0 if pc is None else pc)
+        # supp_constr zeroes where mask == 0, mask_constr averages where mask == 1
+        self.mask = None if mask is None else _mask_u8(mask, lambda a: a != 0, side, dev)
+        if mask_add is None:
+            self.mask_add, self.nmask = None, 0
+        else:
+            ma = np.asarray(mask_add)
+            ma = ma.reshape((-1,) + ma.shape[-2:])
+            self.mask_add = _mask_u8(ma, lambda a: a == 1, side, dev)
+            self.nmask = ma.shape[0]
+        self.work = torch.zeros(max(1, 3 * nslices * ((npix * npix + 2047) // 2048)),
+                                dtype=torch.float64, device=dev)
+
+    @property
+    def needed(self):
+        """Whether the step's fused non-negativity does not cover them."""
+        return (self.mask is not None or self.nmask > 0 or self.pc in (1.0, 2.0)
+                or self.i1 != 0 or self.i2 != self.npix)
+
+    def apply(self, x, x_old=None, stats=None):
+        L.check(L.lib().itomo_sirt_constraints(
+            _p(x), self.npix, x.numel() // (self.npix * self.npix), self.i1, self.i2,
+            _p(self.mask), _p(self.mask_add), self.nmask, self.pc, _p(x_old), _p(stats),
+            _p(self.work), stream_handle()), "sirt_constraints")
+
+
+def sirt(plan, x, b, n_iter, pc=-1.0, check_every=0, mask=None, mask_add=None, region=None,
+         start=0, on_iterate=None):
     """art.py:217-365 on the GPU: R, C weights, then n_iter fused steps
-    x <- x + C*A^T(R*(b - A x)) with phys_constr(x, pc) when pc >= 0.
+    x <- x + C*A^T(R*(b - A x)) with phys_constr(x, pc) when pc >= 0.  With a support
+    ``mask``, additional masks ``mask_add``, pc in {1, 2} (the resolution circle) or a
+    reconstruction ``region`` (i1, i2) smaller than the image (edge padding), the
+    constraints of art.py:303-321 run after each step instead, in the reference's order.
 
     Returns (x, info) with info rows [diff, obj] as in art.py:325-345:
     diff = ||x_{k+1} - x_k|| / ||x_k|| over all slices, obj = ||A x_{k+1} - b||^2 of the
     middle slice.  Stops early on a NaN, like art.py:348-350, when check_every > 0.
+    ``start`` numbers the first step (a run resumed from iterate ``start``);
+    ``on_iterate(k, x)`` is called after step k (k counted from 1 + start), e.g. to write
+    the iterate as art.py:328-341 does with -cit.
     """
     x = _as3(x)
     b = _as3(b)
@@ -56,9 +107,19 @@ def sirt(plan, x, b, n_iter, pc=-1.0, check_every=0):
     R, C = plan.sirt_weights()
     work = torch.empty_like(b)
     hist = torch.zeros((n_iter + 1, 3 * nz), dtype=torch.float64, device=x.device)
+    cons = Constraints(plan.npix, nz, pc, mask, mask_add, region)
+    constrained = cons.needed
+    x_old = torch.empty_like(x) if constrained else None
     done = n_iter
     for it in range(n_iter):
-        plan.sirt_iterate(x, b, R, C, work, pc_beta=pc, stats=hist[it])
+        if constrained:
+            x_old.copy_(x)
+            plan.sirt_iterate(x, b, R, C, work, pc_beta=-1.0, stats=hist[it])
+            cons.apply(x, x_old, hist[it])
+        else:
+            plan.sirt_iterate(x, b, R, C, work, pc_beta=pc, stats=hist[it])
+        if on_iterate is not None:
+            on_iterate(start + it + 1, x)
         if check_every and (it + 1) % check_every == 0:
             if not torch.isfinite(hist[it]).all():
                 done = it + 1

@@ -20,9 +20,49 @@ def sirt_weights(A, At, n, nang, dtype=np.float32):
     return R, C
 
 
-def sirt(x0, b, A, At, n_iter, pc=-1.0, dtype=np.float32):
+def resol_circle(img):
+    """utils.py:229-251 on a 2-D image (a copy)."""
+    m, n = img.shape
+    ctr_x, ctr_y = m * 0.5 - 0.5, n * 0.5 - 0.5
+    d = np.min((ctr_x, ctr_y))
+    xx, yy = np.meshgrid(np.arange(m), np.arange(n))
+    i = np.argwhere(np.sqrt((xx - ctr_x) ** 2 + (yy - ctr_y) ** 2) > d)
+    out = img.copy()
+    out[i[:, 0], i[:, 1]] = 0
+    return out
+
+
+def constrain(x, pc=-1.0, mask=None, mask_add=None, region=None, dtype=np.float32):
+    """art.py:303-321 after a SIRT step on one slice: support mask (utils.py:264-266),
+    additional masks (utils.py:279-281, the float32 numpy mean), then pc (1: x<0 -> 0 and
+    the resolution circle, 2: the circle, other >= 0: x<0 *= pc) on [i1, i2)^2."""
+    x = x.copy()
+    i1, i2 = (0, x.shape[-1]) if region is None else region
+    sub = x[i1:i2, i1:i2]
+    if mask is not None:
+        sub[np.asarray(mask) == 0] = 0.0
+    if mask_add is not None:
+        for m in np.asarray(mask_add).reshape((-1,) + sub.shape):
+            sub[m == 1] = np.mean(sub[m == 1])
+    if pc is not None and pc >= 0.0:
+        if pc == 1.0:
+            neg = sub < 0
+            sub[neg] = sub[neg] * dtype(0.0)
+            sub[:] = resol_circle(sub)
+        if pc == 2.0:
+            sub[:] = resol_circle(sub)
+        else:
+            neg = sub < 0
+            sub[neg] = sub[neg] * dtype(pc)                               # utils.py:171-176
+    x[i1:i2, i1:i2] = sub
+    return x
+
+
+def sirt(x0, b, A, At, n_iter, pc=-1.0, dtype=np.float32, mask=None, mask_add=None,
+         region=None):
     """Returns (x, info, resid) where info rows are [diff, obj] as in art.py:325-345 and
-    resid[k] = ||b - A x_k||_2 for the iterate entering step k (in float64)."""
+    resid[k] = ||b - A x_k||_2 for the iterate entering step k (in float64).  The
+    constraints of art.py:303-321 follow each step (``constrain``)."""
     n = x0.shape[-1]
     nang = b.shape[-2]
     R, C = sirt_weights(A, At, n, nang, dtype)
@@ -35,9 +75,7 @@ def sirt(x0, b, A, At, n_iter, pc=-1.0, dtype=np.float32):
         d = (b - ax).astype(dtype)
         resid.append(np.linalg.norm(d.astype(np.float64)))
         x = (x + C * At((R * d).astype(dtype)).astype(dtype)).astype(dtype)   # art.py:212
-        if pc >= 0.0:
-            neg = x < 0
-            x[neg] = x[neg] * dtype(pc)                                       # utils.py:171-176
+        x = constrain(x, pc, mask, mask_add, region, dtype)
         diff = np.linalg.norm((x - x_old).astype(np.float64)) / np.linalg.norm(
             x_old.astype(np.float64))
         obj = np.linalg.norm((A(x) - b).astype(np.float64)) ** 2
