@@ -110,16 +110,19 @@ def cpu_baseline(img, c_angles, budget_views=20):
                                           t1 - t0, t2 - t1)}
 
 
-def sirt_to_residual(plan, x, b, work, stats, bnorm, target, max_iters, check_every=5):
+def sirt_to_residual(plan, x, b, work, stats, bnorm, target, max_iters, check_every=5,
+                     weights=None, t0=None):
     """Plain SIRT from x (non-negativity on) until ||A x_k - b|| / ||b|| <= target, the
     norms in FP64 on the GPU.  Each step's forward half computes the residual of the
     iterate entering it; every `check_every` steps the host reads it before the back half,
     and the run stops there with x = x_k, so `iterations` is exactly k.  The R/C setup is
-    inside the timed region.  Returns the timing record."""
+    inside the timed region (``weights`` = (R, C) already computed inside it, with the
+    clock started at ``t0``).  Returns the timing record."""
     import torch
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    R, C = plan.sirt_weights()
+    if t0 is None:
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+    R, C = plan.sirt_weights() if weights is None else weights
     it, rel = 0, float("inf")
     while True:
         plan.sirt_forward(x, b, R, work, stats=stats)     # residual of x_it
@@ -132,6 +135,52 @@ def sirt_to_residual(plan, x, b, work, stats, bnorm, target, max_iters, check_ev
     torch.cuda.synchronize()
     return {"target": target, "iterations": it, "seconds": round(time.perf_counter() - t0, 4),
             "reached": rel <= target, "residual": rel, "checked_every": check_every}
+
+
+WARM_WEIGHTS = os.path.join(ROOT, "iterative_tomographic_reconstruction_algorithms_amd",
+                            "weights", "unet_dd_2048x1800.npz")
+
+
+def warm_start_leg(plan, b, bnorm, target, x_plain, plain_seconds, max_iters, input_iters):
+    """The warm-started half of the metric: R/C setup, `input_iters` plain SIRT steps, the
+    trained U-Net on that iterate (with plain SIRT's fallback: a non-finite or no-better
+    output restarts from zero), then plain SIRT continues to the same residual target.
+    Timed end to end (network weights are loaded and the network is run once before the
+    clock starts, as the one-time set-up).  Also the relative L2 distance of its result
+    to the plain leg's result."""
+    import torch
+    from iterative_tomographic_reconstruction_algorithms_amd import warmstart as W
+    net = W.load_weights(WARM_WEIGHTS)
+    n, v = plan.npix, plan.nang
+    dev = b.device
+    work = torch.empty((v, n), dtype=torch.float32, device=dev)
+    stats = torch.zeros(3, dtype=torch.float64, device=dev)
+    net(torch.zeros((1, n, n), device=dev))            # one-time set-up run
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    R, C = plan.sirt_weights()
+    x = torch.zeros((n, n), dtype=torch.float32, device=dev)
+    for _ in range(input_iters):
+        plan.sirt_iterate(x, b, R, C, work, pc_beta=0.0)
+    t1 = torch.cuda.Event(enable_timing=True)
+    t2 = torch.cuda.Event(enable_timing=True)
+    t1.record()
+    x0, fell = W.warm_start(plan, net, b[None], x_in=x[None], pc_beta=0.0)
+    t2.record()
+    x = x0[0].contiguous()
+    rec = sirt_to_residual(plan, x, b, work, stats, bnorm, target, max_iters,
+                           weights=(R, C), t0=t0)
+    rel = float(torch.linalg.vector_norm((x - x_plain).double())
+                / torch.linalg.vector_norm(x_plain.double()))
+    ratio = rec["seconds"] / plain_seconds
+    return {"weights": os.path.relpath(WARM_WEIGHTS, ROOT), "input_iters": input_iters,
+            "continued_iters": rec["iterations"],
+            "iterations": input_iters + rec["iterations"], "seconds": rec["seconds"],
+            "network_and_fallback_ms": round(t1.elapsed_time(t2), 3),
+            "fell_back": bool(fell[0]), "reached": rec["reached"], "residual": rec["residual"],
+            "time_vs_plain": round(ratio, 4), "rel_l2_to_plain": rel,
+            "targets": {"time_vs_plain_max": round(1 / 3, 4), "rel_l2_to_plain_max": 1e-3},
+            "met": {"time": ratio <= 1 / 3, "rel_l2": rel <= 1e-3}}
 
 
 def _events_ms(fn, reps):
@@ -293,6 +342,10 @@ def main():
                          "(a kernel of known traffic for scaling FETCH_SIZE / WRITE_SIZE)")
     ap.add_argument("--extra-configs", action="store_true",
                     help="also measure the other configurations of the suite (minutes)")
+    ap.add_argument("--no-warm-start", dest="warm_start", action="store_false",
+                    help="skip the warm-started leg (runs when the trained weights exist)")
+    ap.add_argument("--warm-input-iters", type=int, default=30,
+                    help="plain SIRT steps before the network (the network's input)")
     ap.add_argument("--max-residual-iters", type=int, default=600,
                     help="also time plain SIRT from zero to the residual target, the R/C "
                          "setup included, for at most this many iterations (0: skip)")
@@ -408,8 +461,13 @@ def main():
     if args.max_residual_iters > 0:
         x.zero_()
         torch.cuda.synchronize()
-        line["plain_sirt_to_residual"] = sirt_to_residual(
-            plan, x, b, work, stats, bnorm, args.residual_target, args.max_residual_iters)
+        plain = sirt_to_residual(plan, x, b, work, stats, bnorm, args.residual_target,
+                                 args.max_residual_iters)
+        line["plain_sirt_to_residual"] = plain
+        if args.warm_start and os.path.exists(WARM_WEIGHTS):
+            line["warm_sirt_to_residual"] = warm_start_leg(
+                plan, b, bnorm, args.residual_target, x.clone(), plain["seconds"],
+                args.max_residual_iters, args.warm_input_iters)
 
     if args.extra_configs:
         line["extra"] = extra_configs(cpu=not args.no_cpu_baseline)
