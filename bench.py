@@ -199,7 +199,8 @@ def _events_ms(fn, reps):
 
 def extra_configs(cpu=True):
     """The other workloads of the benchmark suite, measured on the GPU (opt-in):
-    cfg1 ray-driven SIRT 256^2 x 304 x 100 iterations (with the reference C timed on two
+    cfg1 ray-driven SIRT 256^2 x 304 x 100 iterations (the one-time system-matrix build
+    included; the walks' per-iteration time beside it; the reference C timed on two
     iterations), distance-driven MLEM at 2048^2 x 1800 on Poisson counts, a 64-slice sample of the 1024-slice 1024^2 x 900 stack and the mixed
     256-slice launch through the persistent batch (eager and hipGraph), and CG / ADMM-TV
     per-iteration times at 1024^2 x 720 with 1% Gaussian noise."""
@@ -222,14 +223,23 @@ def extra_configs(cpu=True):
     ang1 = (angles_deg(v1) * np.pi / 180).astype(np.float32)
     p1 = Plan("rd", n1, ang1)
     b1 = p1.forward(torch.from_numpy(img1).to(dev))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    mat_bytes = p1.rd_matrix("build")     # what the first iteration call does by itself
+    t_build = (time.perf_counter() - t0) * 1e3
     R1, C1 = p1.sirt_weights()
     x1 = torch.zeros((n1, n1), device=dev)
     w1 = torch.empty_like(b1)
     st1 = torch.zeros(3, dtype=torch.float64, device=dev)
     t_it = _events_ms(lambda: p1.sirt_iterate(x1, b1, R1, C1, w1, pc_beta=0.0, stats=st1), 100)
     cfg1 = {"workload": "ray-driven SIRT 256x256, 304 views, 100 iterations, x0=0, pc on",
-            "gpu_seconds_100_iters": round(t_it * 100 / 1e3, 4),
-            "gpu_ms_per_iter": round(t_it, 4)}
+            "gpu_seconds_100_iters": round((t_it * 100 + t_build) / 1e3, 4),
+            "gpu_ms_per_iter": round(t_it, 4),
+            "system_matrix_build_ms": round(t_build, 2),
+            "system_matrix_bytes": mat_bytes}
+    p1.rd_matrix("drop")
+    cfg1["gpu_ms_per_iter_walks"] = round(
+        _events_ms(lambda: p1.sirt_iterate(x1, b1, R1, C1, w1, pc_beta=0.0, stats=st1), 20), 4)
     if cpu:
         from oracle import ref, radon as O
         b1h = b1.cpu().numpy()

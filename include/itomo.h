@@ -128,6 +128,18 @@ int itomo_plan_trig(const itomo_plan *plan, float *s_host, float *c_host);
 int itomo_plan_set_shapes(itomo_plan *plan, int fwd_variant, int back_variant);
 int itomo_plan_get_shapes(const itomo_plan *plan, int *fwd_variant, int *back_variant);
 
+/* Ray-driven plans cache the projector's exact system matrix in HBM (both directions,
+ * 8 bytes per walk step, at most 8 GiB together) on their first iteration call and
+ * replay it in every later projection, bitwise equal to the walks.  No counterpart in
+ * the reference (whose radon_rd.c walks every time).  mode: ITOMO_RDM_QUERY reports the
+ * cached bytes (0 if none), ITOMO_RDM_BUILD builds it now (synchronous on stream; 0
+ * bytes when the geometry does not fit), ITOMO_RDM_DROP frees it and keeps the walks.
+ * EINVAL for other projectors. */
+#define ITOMO_RDM_QUERY 0
+#define ITOMO_RDM_BUILD 1
+#define ITOMO_RDM_DROP 2
+int itomo_plan_rd_matrix(itomo_plan *plan, int mode, size_t *bytes, void *stream);
+
 /* Forward projection A (class_projectors_radon.py:60-68): sino = A image.
  * accumulate = 0 overwrites sino, 1 adds to it (the reference C always adds into the
  * zeroed array its wrapper allocates).  Deterministic. */

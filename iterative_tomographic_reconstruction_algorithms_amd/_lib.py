@@ -41,6 +41,7 @@ _SIGS = {
     "itomo_plan_trig": ([_vp, _vp, _vp], _i),
     "itomo_plan_set_shapes": ([_vp, _i, _i], _i),
     "itomo_plan_get_shapes": ([_vp, _vp, _vp], _i),
+    "itomo_plan_rd_matrix": ([_vp, _i, _vp, _vp], _i),
     "itomo_forward": ([_vp, _vp, _vp, _i, _i, _vp], _i),
     "itomo_back": ([_vp, _vp, _vp, _i, _i, _vp], _i),
     "itomo_sirt_weights": ([_vp, _vp, _vp, _vp], _i),

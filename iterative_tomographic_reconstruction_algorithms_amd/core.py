@@ -126,6 +126,16 @@ class Plan:
         L.check(L.lib().itomo_plan_set_shapes(self._h, int(fwd_variant), int(back_variant)),
                 "plan_set_shapes")
 
+    # -- ray-driven system matrix ----------------------------------------------------
+    def rd_matrix(self, mode="query"):
+        """Bytes of the cached ray-driven system matrix (0 if none).  mode "build" builds it
+        now, "drop" frees it and keeps the walks; both give bit-identical projections."""
+        modes = {"query": 0, "build": 1, "drop": 2}
+        nbytes = ctypes.c_size_t()
+        L.check(L.lib().itomo_plan_rd_matrix(self._h, modes[mode], ctypes.byref(nbytes),
+                                             stream_handle()), "plan_rd_matrix")
+        return nbytes.value
+
     # -- projectors -----------------------------------------------------------------
     def forward(self, image, out=None, accumulate=False):
         """A image: (..., N, N) -> (..., V, N)."""

@@ -183,6 +183,8 @@ int projector_forward_epi_any(const Plan& p, int epi, int ones, const float* ima
         return pd_forward_epi(p, epi, ones, image, out, nslices, 0, e, st, bps);
     if (p.projector == ITOMO_PROJ_SS)
         return ss_forward_epi(p, epi, ones, image, out, nslices, 0, e, st, bps);
+    int rc = rd_matrix_ensure(p, st);   // iterations replay the cached system matrix
+    if (rc) return rc;
     return rd_forward_epi(p, epi, ones, image, out, nslices, 0, e, st, bps);
 }
 
@@ -198,8 +200,10 @@ int projector_back_epi_any(const Plan& p, int epi, int ones, const float* sino, 
         return pd_back_epi(p, epi, ones, sino, x, nslices, 0, e, st, bps);
     if (p.projector == ITOMO_PROJ_SS)
         return ss_back_epi(p, epi, ones, sino, x, nslices, 0, e, st, bps);
+    int rc = rd_matrix_ensure(p, st);
+    if (rc) return rc;
     Scratch s;
-    int rc = get_scratch(p, nslices, &s, st);
+    rc = get_scratch(p, nslices, &s, st);
     if (rc) return rc;
     const int64_t n = (int64_t)p.npix * p.npix;
     rc = rd_back_overwrite(p, ones, sino, s.img, nslices, st);

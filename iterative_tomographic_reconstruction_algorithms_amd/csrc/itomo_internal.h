@@ -98,7 +98,29 @@ struct Plan {
     // image).  Grown on first use; a plan must not be used from two streams at once.
     mutable void* scratch;
     mutable size_t scratch_bytes;
+    // Ray-driven plans: the projector's exact system matrix cached in HBM (rd.hip), built
+    // on the first iteration call when it fits kRdMatrixMaxBytes.  Forward rows (one per
+    // bin) and back columns (one per pixel) in chunks of 64 outputs, entry j of lane k of
+    // a chunk at off[chunk] + 64 j + k, cnt[out] = the output's own number of entries; an
+    // entry is {input index, length bits}.
+    mutable int rd_mat_state;   // 0: not tried, 1: built, -1: declined
+    mutable int2* rdf_ent;
+    mutable int64_t* rdf_off;
+    mutable int* rdf_cnt;
+    mutable int2* rdb_ent;
+    mutable int64_t* rdb_off;
+    mutable int* rdb_cnt;
+    mutable size_t rd_mat_bytes;
 };
+
+// Largest ray-driven system matrix a plan caches (forward and back entries together).
+constexpr size_t kRdMatrixMaxBytes = (size_t)8 << 30;
+// Builds the ray-driven system matrix of p if not tried yet and allowed (no caller-owned
+// scratch region active); synchronous.  Returns ITOMO_OK whether built or declined.
+int rd_matrix_ensure(const Plan& p, hipStream_t st);
+void rd_matrix_free(const Plan& p);
+// Whether a caller-owned scratch region is active on this thread (no allocation then).
+bool scratch_region_active();
 
 // Returns plan scratch of at least `bytes` bytes (allocating on first use / growth).
 int plan_scratch(const Plan& p, size_t bytes, void** ptr, hipStream_t st);

@@ -270,6 +270,8 @@ thread_local ScratchRegion t_region = {nullptr, 0};
 
 }  // namespace
 
+bool scratch_region_active() { return t_region.base != nullptr; }
+
 ScratchScope::ScratchScope(ScratchRegion r) : prev_(t_region) { t_region = r; }
 ScratchScope::~ScratchScope() { t_region = prev_; }
 
@@ -446,6 +448,7 @@ int itomo_plan_destroy(itomo_plan* p)
         int st2 = check_hip(hipFree(p->scratch), "plan_destroy: hipFree scratch");
         if (!st) st = st2;
     }
+    rd_matrix_free(*p);
     std::free(p->h_geom);
     delete p;
     return st;

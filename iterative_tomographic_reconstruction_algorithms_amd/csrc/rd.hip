@@ -15,6 +15,9 @@
 // step before it and whether the walk gets there follow from at most six of those
 // parameters, computed exactly as the walk computes them.  Updates reach each pixel in
 // the reference's own order: both projections are bit-identical to the reference's.
+#include <algorithm>
+#include <vector>
+
 #include "epilogue.h"
 
 namespace itomo {
@@ -161,11 +164,61 @@ __device__ float rd_walk(float acc, const float* __restrict__ img_in, int npix,
     return acc;
 }
 
+// The cached system matrix of a plan (see rd_matrix_ensure), or all null.
+struct RdMat {
+    const int2* ent;
+    const int64_t* off;
+    const int* cnt;
+};
+
+// Replays one output's cached entries e[0], e[64], ... (n of them) into acc, in order.
+// The chain of updates is serial (each rounds to float), so the latency of the entry
+// loads and of the gathers they index is hidden by keeping kReplayDepth of each in
+// flight: a batch's gathers and the next batch's entry loads are issued before the
+// batch's updates.  Entries with the top bit set belong to a row-sum or column-sum view
+// and are added plainly.
+constexpr int kReplayDepth = 32;
+
+template <int ARITH, bool ONES, bool BACK>
+__device__ __forceinline__ float rd_replay(float acc, const int2* __restrict__ e, int n,
+                                           const float* __restrict__ src)
+{
+    int2 q[kReplayDepth];
+#pragma unroll
+    for (int u = 0; u < kReplayDepth; ++u)
+        q[u] = u < n ? e[(int64_t)u * 64] : make_int2(0, 0);
+    for (int j = 0; j < n; j += kReplayDepth) {
+        float v[kReplayDepth];
+#pragma unroll
+        for (int u = 0; u < kReplayDepth; ++u)
+            v[u] = (ONES || j + u >= n) ? 1.0f : src[q[u].x & 0x7fffffff];
+        int2 qn[kReplayDepth];
+#pragma unroll
+        for (int u = 0; u < kReplayDepth; ++u)
+            qn[u] = j + kReplayDepth + u < n ? e[(int64_t)(j + kReplayDepth + u) * 64]
+                                             : make_int2(0, 0);
+#pragma unroll
+        for (int u = 0; u < kReplayDepth; ++u) {
+            if (j + u < n) {
+                const float l = __int_as_float(q[u].y);
+                if (q[u].x < 0)
+                    acc = __fadd_rn(acc, v[u]);
+                else if (BACK)
+                    acc = rd_back_update<ARITH>(acc, v[u], l);
+                else
+                    acc = rd_fwd_update<ARITH>(acc, v[u], l);
+            }
+            q[u] = qn[u];
+        }
+    }
+    return acc;
+}
+
 template <int ARITH, int EPI, bool ONES>
 __global__ void __launch_bounds__(64)
 rd_fwd_kernel(const RdView* __restrict__ geom, int npix, const float* __restrict__ image,
               float* __restrict__ out, int64_t img_stride, int64_t sino_stride,
-              int accumulate, EpiArgs e)
+              int accumulate, EpiArgs e, RdMat mat)
 {
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
     const bool active = b < npix;
@@ -177,7 +230,12 @@ rd_fwd_kernel(const RdView* __restrict__ geom, int npix, const float* __restrict
     double p0 = 0.0;
     if (active) {
         float acc = (EPI == EPI_STORE && accumulate) ? out[soff + o] : 0.0f;
-        if (g.kind == RD_COLSUM) {
+        if (mat.ent) {
+            // the bin's cached updates, in order (bitwise the walk's or the sum's)
+            const int64_t chunk = (int64_t)v * gridDim.x + blockIdx.x;
+            acc = rd_replay<ARITH, ONES, false>(acc, mat.ent + mat.off[chunk] + threadIdx.x,
+                                                mat.cnt[(int64_t)v * npix + b], image);
+        } else if (g.kind == RD_COLSUM) {
             const int i = npix - 1 - b;  // bin N-1-i <- column i, i >= 1
             if (i >= 1)
                 for (int j = 0; j < npix; ++j)
@@ -300,6 +358,281 @@ rd_back_gather_kernel(const RdView* __restrict__ geom, int nang, int npix,
     image[o] = pix;
 }
 
+// ---- the cached system matrix ------------------------------------------------------
+//
+// With 288 GB of HBM per GPU the exact system matrix of a small ray-driven geometry fits
+// many times over (8 bytes per walk step, once per direction), and replaying it turns
+// both projections into streaming passes with full occupancy instead of latency-bound
+// walks.  The entries are what the walks compute: per bin the steps of its five sub-rays
+// that enter an image cell, in walk order, {pixel, credited length} (a row-sum or
+// column-sum view's bin: its pixels {pixel | 1 << 31}, in order); per pixel, in view
+// order, the row-sum and column-sum views' bins {bin | 1 << 31} and every sub-ray step
+// entering it {bin, length}.  Replaying them in that order with the same update
+// arithmetic gives the walks' results bit for bit.
+
+// An entry of a row-sum or column-sum view: its input index with the top bit set.
+__device__ __forceinline__ int rd_sum_flag(int index)
+{
+    return (int)((unsigned)index | 0x80000000u);
+}
+
+// One sub-ray walk of radon_rd.c:15-71 (the steps of rd_walk) recording the steps that
+// enter an image cell: FILL == false counts them, FILL == true writes them to out[64 j].
+template <int ARITH, bool FILL>
+__device__ int rd_walk_record(int npix, const float p1[2], const float p2[2], int2* out, int j)
+{
+    const int nh = (int)(npix * 0.5);
+    const float d0 = __fsub_rn(p2[0], p1[0]), d1 = __fsub_rn(p2[1], p1[1]);
+    const float incr = (ARITH == ITOMO_ARITH_FAST)
+                           ? (float)__dsqrt_rn((double)__fmaf_rn(d0, d0, __fmul_rn(d1, d1)))
+                           : (float)__dsqrt_rn((double)__fadd_rn(__fmul_rn(d0, d0),
+                                                                 __fmul_rn(d1, d1)));
+    int ii = (int)rd_round<ARITH>(p1[0]);
+    int jj = (int)rd_round<ARITH>(p1[1]);
+    const float dx = (p2[0] > p1[0]) ? 1.0f : -1.0f;
+    const float dy = (p2[1] > p1[1]) ? 1.0f : -1.0f;
+    const double hx = 0.5 * (double)dx, hy = 0.5 * (double)dy;
+    const double p1x = p1[0], p1y = p1[1], dd0 = d0, dd1 = d1;
+    float alpha = 0.0f, alpha_old = 0.0f;
+    float ax = (float)__ddiv_rn(__dsub_rn(__dadd_rn((double)ii, hx), p1x), dd0);
+    float ay = (float)__ddiv_rn(__dsub_rn(__dadd_rn((double)jj, hy), p1y), dd1);
+    const int cap = 4 * npix + 16;
+    for (int step = 0; alpha <= 1.0f && step < cap; ++step) {
+        if (ax < ay) {
+            alpha = ax;
+            ii = (int)__fadd_rn((float)ii, dx);
+            ax = (float)__ddiv_rn(__dsub_rn(__dadd_rn((double)ii, hx), p1x), dd0);
+        } else if (ax > ay) {
+            alpha = ay;
+            jj = (int)__fadd_rn((float)jj, dy);
+            ay = (float)__ddiv_rn(__dsub_rn(__dadd_rn((double)jj, hy), p1y), dd1);
+        } else if (ax == ay) {
+            alpha = ay;
+            ii = (int)__fadd_rn((float)ii, dx);
+            jj = (int)__fadd_rn((float)jj, dy);
+            ax = (float)__ddiv_rn(__dsub_rn(__dadd_rn((double)ii, hx), p1x), dd0);
+            ay = (float)__ddiv_rn(__dsub_rn(__dadd_rn((double)jj, hy), p1y), dd1);
+        } else {
+            break;
+        }
+        const float l = __fmul_rn(__fsub_rn(alpha, alpha_old), incr);
+        const int i = ii + nh, jc = jj + nh;
+        if (i > 0 && i < npix && jc > 0 && jc < npix) {
+            if (FILL) out[(int64_t)j * 64] = make_int2(i * npix + jc, __float_as_int(l));
+            ++j;
+        }
+        alpha_old = alpha;
+    }
+    return j;
+}
+
+// Forward rows: one thread per (bin, view), as rd_fwd_kernel: the sum views' pixels in the
+// order radon_rd.c:95-115 adds them, else the five sub-rays' steps.
+template <int ARITH, bool FILL>
+__global__ void __launch_bounds__(64)
+rd_fwd_record_kernel(const RdView* __restrict__ geom, int npix, int* __restrict__ cnt,
+                     int2* __restrict__ ent, const int64_t* __restrict__ off)
+{
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    const int v = blockIdx.y;
+    if (b >= npix) return;
+    const RdView g = geom[v];
+    int n = 0;
+    int2* out = FILL ? ent + off[(int64_t)v * gridDim.x + blockIdx.x] + threadIdx.x : nullptr;
+    if (g.kind == RD_COLSUM) {
+        const int i = npix - 1 - b;   // bin N-1-i <- column i, i >= 1
+        if (i >= 1)
+            for (int j = 0; j < npix; ++j, ++n)
+                if (FILL) out[(int64_t)n * 64] = make_int2(rd_sum_flag(j * npix + i), 0);
+    } else if (g.kind == RD_ROWSUM) {
+        for (int j = 0; j < npix; ++j, ++n)
+            if (FILL) out[(int64_t)n * 64] = make_int2(rd_sum_flag(b * npix + j), 0);
+    } else {
+        for (int k = 1; k <= 5; ++k) {
+            float p1[2], p2[2];
+            rd_endpoints<ARITH>(6 * b + k, npix, g, p1, p2);
+            n = rd_walk_record<ARITH, FILL>(npix, p1, p2, out, n);
+        }
+    }
+    if (!FILL) cnt[(int64_t)v * npix + b] = n;
+}
+
+// Back columns: one thread per pixel, the candidates of rd_back_gather_kernel.
+template <int ARITH, bool FILL>
+__global__ void __launch_bounds__(256)
+rd_back_record_kernel(const RdView* __restrict__ geom, int nang, int npix,
+                      int* __restrict__ cnt, int2* __restrict__ ent,
+                      const int64_t* __restrict__ off)
+{
+    const int cc = blockIdx.x * 16 + (threadIdx.x & 15);
+    const int r = blockIdx.y * 16 + (threadIdx.x >> 4);
+    if (r >= npix || cc >= npix) return;
+    const int64_t o = (int64_t)r * npix + cc;
+    int2* out = FILL ? ent + off[o >> 6] + (o & 63) : nullptr;
+    const int nh = (int)(npix * 0.5);
+    const int ii = r - nh, jj = cc - nh;
+    const bool inner = r >= 1 && cc >= 1;
+    int n = 0;
+    for (int v = 0; v < nang; ++v) {
+        const RdView g = geom[v];
+        const int rowbase = v * npix;
+        if (g.kind == RD_COLSUM) {
+            if (cc >= 1) {
+                if (FILL) out[(int64_t)n * 64] = make_int2(rd_sum_flag(rowbase + npix - 1 - cc), 0);
+                ++n;
+            }
+        } else if (g.kind == RD_ROWSUM) {
+            if (FILL) out[(int64_t)n * 64] = make_int2(rd_sum_flag(rowbase + r), 0);
+            ++n;
+        } else if (inner) {
+            const double sd = (double)g.s, cd = (double)g.c;
+            const double tc = (double)jj * cd - (double)ii * sd;
+            const double hw = 0.5 * (fabs(cd) + fabs(sd));
+            int ilo = (int)floor(6.0 * ((double)nh - tc - hw)) - 1;
+            int ihi = (int)ceil(6.0 * ((double)nh - tc + hw)) + 1;
+            if ((npix & 1) && tc - hw - 1.0 / 3.0 <= 2.0 / 6.0 && tc + hw + 1.0 / 3.0 >= 1.0 / 6.0) {
+                ilo = min(ilo, 6 * nh + 1);
+                ihi = max(ihi, 6 * nh + 2);
+            }
+            ilo = max(ilo, 1);
+            ihi = min(ihi, 6 * npix - 1);
+            for (int i = ilo; i <= ihi; ++i) {
+                if (i % 6 == 0) continue;
+                float p1[2], p2[2], l;
+                rd_endpoints<ARITH>(i, npix, g, p1, p2);
+                if (!rd_cell_step<ARITH>(p1, p2, ii, jj, &l)) continue;
+                if (FILL) out[(int64_t)n * 64] = make_int2(rowbase + i / 6, __float_as_int(l));
+                ++n;
+            }
+        }
+    }
+    if (!FILL) cnt[o] = n;
+}
+
+// Back projection from the cached columns: one thread per pixel, updates in view order
+// with rd_back_gather_kernel's arithmetic.
+template <int ARITH, bool ONES>
+__global__ void __launch_bounds__(256)
+rd_back_matrix_kernel(const int2* __restrict__ ent, const int64_t* __restrict__ off,
+                      const int* __restrict__ cnt, int npix, const float* __restrict__ sino,
+                      float* __restrict__ image, int64_t sino_stride, int64_t img_stride,
+                      int accumulate)
+{
+    const int64_t o = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int64_t n2 = (int64_t)npix * npix;
+    if (o >= n2) return;
+    image += (int64_t)blockIdx.y * img_stride;
+    sino += (int64_t)blockIdx.y * sino_stride;
+    const float pix = accumulate ? image[o] : 0.0f;
+    image[o] = rd_replay<ARITH, ONES, true>(pix, ent + off[o >> 6] + (o & 63), cnt[o], sino);
+}
+
+namespace {
+
+// Chunk offsets of a matrix from its per-output counts: outputs are grouped 64 at a time
+// within rows of `row` outputs, and a chunk holds 64 x (its largest count) entries.
+// Returns the number of entries.
+int64_t chunk_layout(const std::vector<int>& cnt, int64_t nout, int64_t chunks_per_row,
+                     int64_t row, std::vector<int64_t>* off)
+{
+    const int64_t nrows = nout / row;
+    off->assign((size_t)(nrows * chunks_per_row), 0);
+    int64_t total = 0;
+    for (int64_t rr = 0; rr < nrows; ++rr)
+        for (int64_t c = 0; c < chunks_per_row; ++c) {
+            int m = 0;
+            for (int64_t k = c * 64; k < std::min<int64_t>(row, c * 64 + 64); ++k)
+                m = std::max(m, cnt[(size_t)(rr * row + k)]);
+            (*off)[(size_t)(rr * chunks_per_row + c)] = total;
+            total += (int64_t)m * 64;
+        }
+    return total;
+}
+
+}  // namespace
+
+void rd_matrix_free(const Plan& p)
+{
+    void* ptrs[] = {p.rdf_ent, p.rdf_off, p.rdf_cnt, p.rdb_ent, p.rdb_off, p.rdb_cnt};
+    for (void* q : ptrs)
+        if (q) (void)hipFree(q);
+    p.rdf_ent = p.rdb_ent = nullptr;
+    p.rdf_off = p.rdb_off = nullptr;
+    p.rdf_cnt = p.rdb_cnt = nullptr;
+    p.rd_mat_bytes = 0;
+}
+
+template <int ARITH>
+static int rd_matrix_build(const Plan& p, hipStream_t st)
+{
+    const int n = p.npix, V = p.nang;
+    const int64_t n2 = (int64_t)n * n, nf = (int64_t)V * n;
+    const int fcpr = (n + 63) / 64;                       // forward chunks per view
+    const RdView* g = (const RdView*)p.d_geom;
+    int rc = check_hip(hipMalloc(&p.rdf_cnt, nf * sizeof(int)), "rd matrix: counts");
+    if (!rc) rc = check_hip(hipMalloc(&p.rdb_cnt, n2 * sizeof(int)), "rd matrix: counts");
+    if (rc) return rc;
+    rd_fwd_record_kernel<ARITH, false><<<dim3(fcpr, V), 64, 0, st>>>(g, n, p.rdf_cnt, nullptr,
+                                                                       nullptr);
+    rd_back_record_kernel<ARITH, false><<<dim3((n + 15) / 16, (n + 15) / 16), 256, 0, st>>>(
+        g, V, n, p.rdb_cnt, nullptr, nullptr);
+    rc = check_hip(hipGetLastError(), "rd matrix: count launch");
+    std::vector<int> cf((size_t)nf), cb((size_t)n2);
+    if (!rc) rc = check_hip(hipMemcpyAsync(cf.data(), p.rdf_cnt, nf * sizeof(int),
+                                           hipMemcpyDeviceToHost, st), "rd matrix: counts");
+    if (!rc) rc = check_hip(hipMemcpyAsync(cb.data(), p.rdb_cnt, n2 * sizeof(int),
+                                           hipMemcpyDeviceToHost, st), "rd matrix: counts");
+    if (!rc) rc = check_hip(hipStreamSynchronize(st), "rd matrix: sync");
+    if (rc) return rc;
+    std::vector<int64_t> of, ob;
+    const int64_t tf = chunk_layout(cf, nf, fcpr, n, &of);
+    // back: pixels in row-major chunks of 64 (one "row" of all n2 outputs)
+    const int64_t tb = chunk_layout(cb, n2, (n2 + 63) / 64, n2, &ob);
+    const size_t bytes = (size_t)(tf + tb) * sizeof(int2);
+    if (bytes > kRdMatrixMaxBytes) return ITOMO_OK;      // declined: the walks stay
+    rc = check_hip(hipMalloc(&p.rdf_ent, std::max<int64_t>(tf, 1) * sizeof(int2)), "rd matrix");
+    if (!rc) rc = check_hip(hipMalloc(&p.rdb_ent, std::max<int64_t>(tb, 1) * sizeof(int2)), "rd matrix");
+    if (!rc) rc = check_hip(hipMalloc(&p.rdf_off, of.size() * sizeof(int64_t)), "rd matrix");
+    if (!rc) rc = check_hip(hipMalloc(&p.rdb_off, ob.size() * sizeof(int64_t)), "rd matrix");
+    if (!rc) rc = check_hip(hipMemcpy(p.rdf_off, of.data(), of.size() * sizeof(int64_t),
+                                      hipMemcpyHostToDevice), "rd matrix");
+    if (!rc) rc = check_hip(hipMemcpy(p.rdb_off, ob.data(), ob.size() * sizeof(int64_t),
+                                      hipMemcpyHostToDevice), "rd matrix");
+    if (rc) return rc;
+    rd_fwd_record_kernel<ARITH, true><<<dim3(fcpr, V), 64, 0, st>>>(g, n, nullptr, p.rdf_ent,
+                                                                      p.rdf_off);
+    rd_back_record_kernel<ARITH, true><<<dim3((n + 15) / 16, (n + 15) / 16), 256, 0, st>>>(
+        g, V, n, nullptr, p.rdb_ent, p.rdb_off);
+    rc = check_hip(hipGetLastError(), "rd matrix: fill launch");
+    if (!rc) rc = check_hip(hipStreamSynchronize(st), "rd matrix: sync");
+    if (rc) return rc;
+    p.rd_mat_bytes = bytes;
+    p.rd_mat_state = 1;
+    return ITOMO_OK;
+}
+
+int rd_matrix_ensure(const Plan& p, hipStream_t st)
+{
+    if (p.projector != ITOMO_PROJ_RD || p.rd_mat_state != 0 || scratch_region_active())
+        return ITOMO_OK;
+    // no synchronising build under stream capture: the walks are captured instead
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    int rc = check_hip(hipStreamIsCapturing(st, &cs), "rd matrix: capture status");
+    if (rc || cs != hipStreamCaptureStatusNone) return rc;
+    p.rd_mat_state = -1;
+    // upper bound of the entries (a walk takes at most 2 npix + 2 steps): skip the counting
+    // passes when even a fraction of it cannot fit
+    const double bound = 16.0 * 5.0 * p.nang * (double)p.npix * (1.5 * p.npix);
+    if (bound > 4.0 * (double)kRdMatrixMaxBytes) return ITOMO_OK;
+    // entry indices are 31-bit (the top bit flags a sum view's bin)
+    if ((int64_t)p.nang * p.npix >= ((int64_t)1 << 31)) return ITOMO_OK;
+    rc = check_hip(hipStreamSynchronize(st), "rd matrix: sync");
+    if (!rc) rc = p.arith == ITOMO_ARITH_FAST ? rd_matrix_build<ITOMO_ARITH_FAST>(p, st)
+                                              : rd_matrix_build<ITOMO_ARITH_IEEE>(p, st);
+    if (p.rd_mat_state != 1) rd_matrix_free(p);
+    return rc;
+}
+
 namespace {
 
 template <int ARITH, int EPI, bool ONES>
@@ -307,9 +640,12 @@ void launch_rd_fwd(const Plan& p, const float* image, float* out, int nslices,
                    int accumulate, const EpiArgs& e, hipStream_t st)
 {
     dim3 block(64), grid((p.npix + 63) / 64, p.nang, nslices);
+    const RdMat mat = p.rd_mat_state == 1
+                          ? RdMat{p.rdf_ent, p.rdf_off, p.rdf_cnt}
+                          : RdMat{nullptr, nullptr, nullptr};
     rd_fwd_kernel<ARITH, EPI, ONES><<<grid, block, 0, st>>>(
         (const RdView*)p.d_geom, p.npix, image, out, (int64_t)p.npix * p.npix,
-        (int64_t)p.nang * p.npix, accumulate, e);
+        (int64_t)p.nang * p.npix, accumulate, e, mat);
 }
 
 template <int ARITH>
@@ -359,6 +695,21 @@ static int rd_back_launch(const Plan& p, int ones, const float* sino, float* ima
                           int64_t sino_stride, int64_t img_stride, int accumulate,
                           hipStream_t st)
 {
+    if (p.rd_mat_state == 1) {
+        const int64_t n2 = (int64_t)p.npix * p.npix;
+        dim3 mg((unsigned)((n2 + 255) / 256), nslices);
+#define ITOMO_RDM(A, O)                                                                  \
+    rd_back_matrix_kernel<A, O><<<mg, 256, 0, st>>>(p.rdb_ent, p.rdb_off, p.rdb_cnt, p.npix, \
+                                                   sino, image, sino_stride, img_stride,  \
+                                                   accumulate)
+        if (p.arith == ITOMO_ARITH_FAST) {
+            if (ones) ITOMO_RDM(ITOMO_ARITH_FAST, true); else ITOMO_RDM(ITOMO_ARITH_FAST, false);
+        } else {
+            if (ones) ITOMO_RDM(ITOMO_ARITH_IEEE, true); else ITOMO_RDM(ITOMO_ARITH_IEEE, false);
+        }
+#undef ITOMO_RDM
+        return check_hip(hipGetLastError(), "rd back (matrix) launch");
+    }
     dim3 block(256), grid((p.npix + 15) / 16, (p.npix + 15) / 16, nslices);
     const RdView* g = (const RdView*)p.d_geom;
     if (p.arith == ITOMO_ARITH_FAST) {
@@ -400,3 +751,23 @@ int rd_back(const Plan& p, const float* sino, float* image, int nslices, int64_t
 }
 
 }  // namespace itomo
+
+extern "C" int itomo_plan_rd_matrix(itomo_plan* plan, int mode, size_t* bytes, void* stream)
+{
+    if (!plan || plan->projector != ITOMO_PROJ_RD || mode < ITOMO_RDM_QUERY ||
+        mode > ITOMO_RDM_DROP)
+        return itomo::set_error(ITOMO_EINVAL, "plan_rd_matrix: bad argument");
+    int rc = ITOMO_OK;
+    if (mode == ITOMO_RDM_BUILD) {
+        if (itomo::scratch_region_active())
+            return itomo::set_error(ITOMO_EINVAL, "plan_rd_matrix: inside a no-allocation call");
+        if (plan->rd_mat_state == -1) plan->rd_mat_state = 0;
+        rc = itomo::rd_matrix_ensure(*plan, (hipStream_t)stream);
+    } else if (mode == ITOMO_RDM_DROP) {
+        rc = itomo::check_hip(hipStreamSynchronize((hipStream_t)stream), "plan_rd_matrix: sync");
+        itomo::rd_matrix_free(*plan);
+        plan->rd_mat_state = -1;
+    }
+    if (bytes) *bytes = plan->rd_mat_state == 1 ? plan->rd_mat_bytes : 0;
+    return rc;
+}
