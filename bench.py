@@ -197,7 +197,7 @@ def _events_ms(fn, reps):
     return a.elapsed_time(b) / reps
 
 
-def extra_configs(cpu=True):
+def extra_configs(cpu=True, which=("cfg1", "cfg2", "cfg3", "cfg4")):
     """The other workloads of the benchmark suite, measured on the GPU (opt-in):
     cfg1 ray-driven SIRT 256^2 x 304 x 100 iterations (the one-time system-matrix build
     included; the walks' per-iteration time beside it; the reference C timed on two
@@ -217,126 +217,130 @@ def extra_configs(cpu=True):
     def dd_plan(n, v):
         return Plan("dd", n, np.fft.fftshift((angles_deg(v) * np.pi / 180).astype(np.float32)))
 
-    # cfg1: ray-driven SIRT, 100 iterations from zero, non-negativity on
-    n1, v1 = 256, 304
-    img1 = ellipses(n1, 10, seed=1)
-    ang1 = (angles_deg(v1) * np.pi / 180).astype(np.float32)
-    p1 = Plan("rd", n1, ang1)
-    b1 = p1.forward(torch.from_numpy(img1).to(dev))
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    mat_bytes = p1.rd_matrix("build")     # what the first iteration call does by itself
-    t_build = (time.perf_counter() - t0) * 1e3
-    R1, C1 = p1.sirt_weights()
-    x1 = torch.zeros((n1, n1), device=dev)
-    w1 = torch.empty_like(b1)
-    st1 = torch.zeros(3, dtype=torch.float64, device=dev)
-    t_it = _events_ms(lambda: p1.sirt_iterate(x1, b1, R1, C1, w1, pc_beta=0.0, stats=st1), 100)
-    cfg1 = {"workload": "ray-driven SIRT 256x256, 304 views, 100 iterations, x0=0, pc on",
-            "gpu_seconds_100_iters": round((t_it * 100 + t_build) / 1e3, 4),
-            "gpu_ms_per_iter": round(t_it, 4),
-            "system_matrix_build_ms": round(t_build, 2),
-            "system_matrix_bytes": mat_bytes}
-    p1.rd_matrix("drop")
-    cfg1["gpu_ms_per_iter_walks"] = round(
-        _events_ms(lambda: p1.sirt_iterate(x1, b1, R1, C1, w1, pc_beta=0.0, stats=st1), 20), 4)
-    if cpu:
-        from oracle import ref, radon as O
-        b1h = b1.cpu().numpy()
-        x1h = np.zeros((n1, n1), np.float32)
-        use_ref = ref.available("rd", "fast")
-        fw = (lambda z: ref.forward("rd", z, ang1, "fast")) if use_ref else \
-             (lambda z: O.rd_forward(z, ang1, O.FAST))
-        bw = (lambda z: ref.back("rd", z, ang1, "fast")) if use_ref else \
-             (lambda z: O.rd_back(z, ang1, O.FAST))
+    if "cfg1" in which:
+        # cfg1: ray-driven SIRT, 100 iterations from zero, non-negativity on
+        n1, v1 = 256, 304
+        img1 = ellipses(n1, 10, seed=1)
+        ang1 = (angles_deg(v1) * np.pi / 180).astype(np.float32)
+        p1 = Plan("rd", n1, ang1)
+        b1 = p1.forward(torch.from_numpy(img1).to(dev))
+        torch.cuda.synchronize()
         t0 = time.perf_counter()
-        for _ in range(2):   # one SIRT iteration = forward, back, monitoring forward
-            r = b1h - fw(x1h)
-            x1h = x1h + bw(r)
-            fw(x1h)
-        secs = (time.perf_counter() - t0) / 2
-        cfg1["cpu_seconds_100_iters_extrapolated"] = round(secs * 100, 2)
-        cfg1["cpu_kind"] = "reference" if use_ref else "port"
-        cfg1["cpu_sample"] = "2 iterations timed single-threaded, x50"
-    out["cfg1_rd_sirt_256x304"] = cfg1
+        mat_bytes = p1.rd_matrix("build")     # what the first iteration call does by itself
+        t_build = (time.perf_counter() - t0) * 1e3
+        R1, C1 = p1.sirt_weights()
+        x1 = torch.zeros((n1, n1), device=dev)
+        w1 = torch.empty_like(b1)
+        st1 = torch.zeros(3, dtype=torch.float64, device=dev)
+        t_it = _events_ms(lambda: p1.sirt_iterate(x1, b1, R1, C1, w1, pc_beta=0.0, stats=st1), 100)
+        cfg1 = {"workload": "ray-driven SIRT 256x256, 304 views, 100 iterations, x0=0, pc on",
+                "gpu_seconds_100_iters": round((t_it * 100 + t_build) / 1e3, 4),
+                "gpu_ms_per_iter": round(t_it, 4),
+                "system_matrix_build_ms": round(t_build, 2),
+                "system_matrix_bytes": mat_bytes}
+        p1.rd_matrix("drop")
+        cfg1["gpu_ms_per_iter_walks"] = round(
+            _events_ms(lambda: p1.sirt_iterate(x1, b1, R1, C1, w1, pc_beta=0.0, stats=st1), 20), 4)
+        if cpu:
+            from oracle import ref, radon as O
+            b1h = b1.cpu().numpy()
+            x1h = np.zeros((n1, n1), np.float32)
+            use_ref = ref.available("rd", "fast")
+            fw = (lambda z: ref.forward("rd", z, ang1, "fast")) if use_ref else \
+                 (lambda z: O.rd_forward(z, ang1, O.FAST))
+            bw = (lambda z: ref.back("rd", z, ang1, "fast")) if use_ref else \
+                 (lambda z: O.rd_back(z, ang1, O.FAST))
+            t0 = time.perf_counter()
+            for _ in range(2):   # one SIRT iteration = forward, back, monitoring forward
+                r = b1h - fw(x1h)
+                x1h = x1h + bw(r)
+                fw(x1h)
+            secs = (time.perf_counter() - t0) / 2
+            cfg1["cpu_seconds_100_iters_extrapolated"] = round(secs * 100, 2)
+            cfg1["cpu_kind"] = "reference" if use_ref else "port"
+            cfg1["cpu_sample"] = "2 iterations timed single-threaded, x50"
+        out["cfg1_rd_sirt_256x304"] = cfg1
 
-    # cfg2 MLEM: Poisson counts I0 * A x / max(A x) with I0 = 1e4, from ones
-    n2, v2 = 2048, 1800
-    p2 = dd_plan(n2, v2)
-    from iterative_tomographic_reconstruction_algorithms_amd import autotune
-    autotune.tune(p2, nslices=1, reps=2, use_cache=False)
-    ax2 = p2.forward(torch.from_numpy(ellipses(n2, 40, seed=2)).to(dev))
-    gen = torch.Generator(device=dev).manual_seed(2)
-    b2 = torch.poisson(1e4 * ax2 / ax2.max(), generator=gen)
-    inv_d2 = p2.mlem_weights()
-    x2 = torch.ones((n2, n2), device=dev)
-    w2 = torch.empty_like(b2)
-    st2 = torch.zeros(3, dtype=torch.float64, device=dev)
-    t_m = _events_ms(lambda: p2.mlem_iterate(x2, b2, inv_d2, w2, stats=st2), 20)
-    out["cfg2_dd_mlem_2048x1800"] = {
-        "workload": "distance-driven MLEM 2048x2048, 1800 views, Poisson counts (I0 = 1e4), x0 = 1",
-        "gpu_ms_per_iter": round(t_m, 4),
-        "gupdates_per_s": round(2.0 * n2 * n2 * v2 / (t_m * 1e-3) / 1e9, 1),
-        "gpu_seconds_200_iters": round(t_m * 200 / 1e3, 3)}
-    del p2, ax2, b2, inv_d2, x2, w2
+    if "cfg2" in which:
+        # cfg2 MLEM: Poisson counts I0 * A x / max(A x) with I0 = 1e4, from ones
+        n2, v2 = 2048, 1800
+        p2 = dd_plan(n2, v2)
+        from iterative_tomographic_reconstruction_algorithms_amd import autotune
+        autotune.tune(p2, nslices=1, reps=2, use_cache=False)
+        ax2 = p2.forward(torch.from_numpy(ellipses(n2, 40, seed=2)).to(dev))
+        gen = torch.Generator(device=dev).manual_seed(2)
+        b2 = torch.poisson(1e4 * ax2 / ax2.max(), generator=gen)
+        inv_d2 = p2.mlem_weights()
+        x2 = torch.ones((n2, n2), device=dev)
+        w2 = torch.empty_like(b2)
+        st2 = torch.zeros(3, dtype=torch.float64, device=dev)
+        t_m = _events_ms(lambda: p2.mlem_iterate(x2, b2, inv_d2, w2, stats=st2), 20)
+        out["cfg2_dd_mlem_2048x1800"] = {
+            "workload": "distance-driven MLEM 2048x2048, 1800 views, Poisson counts (I0 = 1e4), x0 = 1",
+            "gpu_ms_per_iter": round(t_m, 4),
+            "gupdates_per_s": round(2.0 * n2 * n2 * v2 / (t_m * 1e-3) / 1e9, 1),
+            "gpu_seconds_200_iters": round(t_m * 200 / 1e3, 3)}
+        del p2, ax2, b2, inv_d2, x2, w2
 
-    # cfg3a: 64 of the 1024 slices of a 1024^2 x 900 ellipsoid stack, one persistent batch
-    n3, v3, nz3 = 1024, 900, 64
-    p3 = dd_plan(n3, v3)
-    vol = torch.from_numpy(ellipsoid_slices(n3, 1024, 60, seed=3000, first=480, count=nz3))
-    b3 = p3.forward(vol.to(dev))
-    del vol
-    m3 = Member(p3, b3)
-    bt = Batch([m3], pc_beta=0.0)
-    t3 = _events_ms(bt.iterate, 5)
-    g3 = bt.graph(iters=1)
-    t3g = _events_ms(g3.launch, 5)
-    g3.close()
-    ups = 2.0 * n3 * n3 * v3 * nz3
-    out["cfg3_stack_1024x900"] = {
-        "workload": "persistent batched SIRT, 64 of 1024 slices of 1024^2 x 900 views",
-        "ms_per_iter": round(t3, 3), "ms_per_iter_graph": round(t3g, 3),
-        "gupdates_per_s": round(ups / t3 / 1e6, 1),
-        "seconds_100_iters_1024_slices_extrapolated": round(t3 * 16 * 100 / 1e3, 2)}
-    bt.close()
-    del m3, b3
+    if "cfg3" in which:
+        # cfg3a: 64 of the 1024 slices of a 1024^2 x 900 ellipsoid stack, one persistent batch
+        n3, v3, nz3 = 1024, 900, 64
+        p3 = dd_plan(n3, v3)
+        vol = torch.from_numpy(ellipsoid_slices(n3, 1024, 60, seed=3000, first=480, count=nz3))
+        b3 = p3.forward(vol.to(dev))
+        del vol
+        m3 = Member(p3, b3)
+        bt = Batch([m3], pc_beta=0.0)
+        t3 = _events_ms(bt.iterate, 5)
+        g3 = bt.graph(iters=1)
+        t3g = _events_ms(g3.launch, 5)
+        g3.close()
+        ups = 2.0 * n3 * n3 * v3 * nz3
+        out["cfg3_stack_1024x900"] = {
+            "workload": "persistent batched SIRT, 64 of 1024 slices of 1024^2 x 900 views",
+            "ms_per_iter": round(t3, 3), "ms_per_iter_graph": round(t3g, 3),
+            "gupdates_per_s": round(ups / t3 / 1e6, 1),
+            "seconds_100_iters_1024_slices_extrapolated": round(t3 * 16 * 100 / 1e3, 2)}
+        bt.close()
+        del m3, b3
 
-    # cfg3b: one mixed launch of 256 slices, sizes x view counts, 16 slices each
-    members, ups = [], 0.0
-    for n in (256, 512, 1024, 2048):
-        for v in (180, 360, 720, 1440):
-            p = dd_plan(n, v)
-            imgs = torch.from_numpy(np.stack([ellipses(n, 10, seed=k) for k in range(2)]))
-            b = p.forward(imgs.to(dev)).repeat(8, 1, 1).contiguous()
-            members.append(Member(p, b))
-            ups += 2.0 * n * n * v * 16
-    bm = Batch(members, pc_beta=0.0)
-    tm = _events_ms(bm.iterate, 3)
-    gm = bm.graph(iters=1)
-    tmg = _events_ms(gm.launch, 3)
-    gm.close()
-    out["cfg3_mixed_256_slices"] = {
-        "workload": "one persistent launch: 16 slices each of {256,512,1024,2048}^2 x "
-                    "{180,360,720,1440} views", "ms_per_iter": round(tm, 3),
-        "ms_per_iter_graph": round(tmg, 3), "gupdates_per_s": round(ups / tm / 1e6, 1)}
-    bm.close()
-    del members
+        # cfg3b: one mixed launch of 256 slices, sizes x view counts, 16 slices each
+        members, ups = [], 0.0
+        for n in (256, 512, 1024, 2048):
+            for v in (180, 360, 720, 1440):
+                p = dd_plan(n, v)
+                imgs = torch.from_numpy(np.stack([ellipses(n, 10, seed=k) for k in range(2)]))
+                b = p.forward(imgs.to(dev)).repeat(8, 1, 1).contiguous()
+                members.append(Member(p, b))
+                ups += 2.0 * n * n * v * 16
+        bm = Batch(members, pc_beta=0.0)
+        tm = _events_ms(bm.iterate, 3)
+        gm = bm.graph(iters=1)
+        tmg = _events_ms(gm.launch, 3)
+        gm.close()
+        out["cfg3_mixed_256_slices"] = {
+            "workload": "one persistent launch: 16 slices each of {256,512,1024,2048}^2 x "
+                        "{180,360,720,1440} views", "ms_per_iter": round(tm, 3),
+            "ms_per_iter_graph": round(tmg, 3), "gupdates_per_s": round(ups / tm / 1e6, 1)}
+        bm.close()
+        del members
 
-    # cfg4: CG and ADMM-TV at 1024^2 x 720, Gaussian noise of 1% of the largest value
-    n4, v4 = 1024, 720
-    p4 = dd_plan(n4, v4)
-    b4 = p4.forward(torch.from_numpy(ellipses(n4, 10, seed=4)).to(dev))
-    gen = torch.Generator(device=dev).manual_seed(4)
-    b4 = (b4 + 0.01 * float(b4.max()) * torch.randn(b4.shape, device=dev, generator=gen)).contiguous()
-    x4 = torch.zeros((n4, n4), device=dev)
-    t_cg = _events_ms(lambda: solvers.cg(p4, x4.clone(), b4, 10), 1) / 10
-    t_admm = _events_ms(lambda: solvers.admm_tv(p4, x4.clone(), b4, 5, n_inner=4), 1) / 5
-    out["cfg4_cg_admm_1024x720"] = {
-        "workload": "CG and ADMM-TV (4 inner CG steps per outer step), 1024^2 x 720, "
-                    "1% Gaussian noise, FP64 scalars on the GPU",
-        "cg_ms_per_iter": round(t_cg, 3), "admm_ms_per_outer_iter": round(t_admm, 3),
-        "cg_seconds_300_iters": round(t_cg * 0.3, 2),
-        "admm_seconds_300_iters": round(t_admm * 0.3, 2)}
+    if "cfg4" in which:
+        # cfg4: CG and ADMM-TV at 1024^2 x 720, Gaussian noise of 1% of the largest value
+        n4, v4 = 1024, 720
+        p4 = dd_plan(n4, v4)
+        b4 = p4.forward(torch.from_numpy(ellipses(n4, 10, seed=4)).to(dev))
+        gen = torch.Generator(device=dev).manual_seed(4)
+        b4 = (b4 + 0.01 * float(b4.max()) * torch.randn(b4.shape, device=dev, generator=gen)).contiguous()
+        x4 = torch.zeros((n4, n4), device=dev)
+        t_cg = _events_ms(lambda: solvers.cg(p4, x4.clone(), b4, 10), 1) / 10
+        t_admm = _events_ms(lambda: solvers.admm_tv(p4, x4.clone(), b4, 5, n_inner=4), 1) / 5
+        out["cfg4_cg_admm_1024x720"] = {
+            "workload": "CG and ADMM-TV (4 inner CG steps per outer step), 1024^2 x 720, "
+                        "1% Gaussian noise, FP64 scalars on the GPU",
+            "cg_ms_per_iter": round(t_cg, 3), "admm_ms_per_outer_iter": round(t_admm, 3),
+            "cg_seconds_300_iters": round(t_cg * 0.3, 2),
+            "admm_seconds_300_iters": round(t_admm * 0.3, 2)}
     return out
 
 
@@ -352,6 +356,8 @@ def main():
                          "(a kernel of known traffic for scaling FETCH_SIZE / WRITE_SIZE)")
     ap.add_argument("--extra-configs", action="store_true",
                     help="also measure the other configurations of the suite (minutes)")
+    ap.add_argument("--extra-only", default="cfg1,cfg2,cfg3,cfg4",
+                    help="with --extra-configs: the configurations to run (comma list)")
     ap.add_argument("--no-warm-start", dest="warm_start", action="store_false",
                     help="skip the warm-started leg (runs when the trained weights exist)")
     ap.add_argument("--warm-input-iters", type=int, default=30,
@@ -480,7 +486,8 @@ def main():
                 args.max_residual_iters, args.warm_input_iters)
 
     if args.extra_configs:
-        line["extra"] = extra_configs(cpu=not args.no_cpu_baseline)
+        line["extra"] = extra_configs(cpu=not args.no_cpu_baseline,
+                                      which=tuple(args.extra_only.split(",")))
 
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(img, c_angles)

@@ -33,8 +33,9 @@ pytestmark = pytest.mark.gpu
 
 
 def rel(a, b):
-    a = np.asarray(a, np.float64)
-    b = np.asarray(b, np.float64)
+    cplx = np.iscomplexobj(a) or np.iscomplexobj(b)
+    a = np.asarray(a, np.complex128 if cplx else np.float64)
+    b = np.asarray(b, np.complex128 if cplx else np.float64)
     return np.linalg.norm(a - b) / np.linalg.norm(b)
 
 
