@@ -408,20 +408,23 @@ __device__ __forceinline__ void load_run(__amdgpu_buffer_rsrc_t rs, int idx, int
 }
 
 // One candidate pixel of a bin [Dlo, Dhi] on a line with offset K: the pixel spans
-// [Plo, x+1-K].  Only the first candidate can start left of Dlo.  Dhc is the bin's right
-// end clipped to the line's L*.  Advances x and Plo.  A candidate that misses the bin
-// gets the weight max(R - L, 0) = 0, which leaves the sum unchanged (every candidate
-// value is a finite image value or a zero), so no overlap test is needed.
+// [x-K, x+1-K].  Dhc is the bin's right end clipped to the line's L*.  Advances x.
+// The first candidate (Rp = x-K, its left end, on entry) can start left of Dlo and gets
+// the weight max(min(P_hi, Dhc) - max(P_lo, Dlo), 0).  Every later one gets
+// R_k - R_{k-1} with R_k = min(P_hi(k), Dhc), carried in Rp: its left end is the
+// previous candidate's right end P_hi(k-1), and where P_hi(k-1) < Dhc that is the same
+// subtraction as max(min(P_hi(k), Dhc) - P_hi(k-1), 0), while past the bin's end both
+// give +0 (R_k = R_{k-1} = Dhc) -- so no clamp is needed.  A weight of +0 leaves the
+// sum unchanged (every candidate value is a finite image value or a zero).
 template <int ARITH>
-__device__ __forceinline__ float cand_step(bool first, float& x, float& Plo, float K,
+__device__ __forceinline__ float cand_step(bool first, float& x, float& Rp, float K,
                                            float Dlo, float Dhc, float val, float acc)
 {
     x = __fadd_rn(x, 1.0f);
-    const float Phi = __fsub_rn(x, K);
-    const float R = fminf(Phi, Dhc);
-    const float L = first ? fmaxf(Plo, Dlo) : Plo;
-    Plo = Phi;
-    return accum<ARITH>(fmaxf(__fsub_rn(R, L), 0.0f), val, acc);
+    const float R = fminf(__fsub_rn(x, K), Dhc);
+    const float w = first ? fmaxf(__fsub_rn(R, fmaxf(Rp, Dlo)), 0.0f) : __fsub_rn(R, Rp);
+    Rp = R;
+    return accum<ARITH>(w, val, acc);
 }
 
 // (index - nh) of the pixel whose span [P_q, P_q+1) holds Dlo on a line with offset K,
@@ -492,10 +495,10 @@ __device__ __forceinline__ float fwd_grouped(const __amdgpu_buffer_rsrc_t rs, co
         for (int l = 0; l < GL; ++l) {
             float Dhc = Dhi;
             if (last_bin || Dhi > __fsub_rn(xNm1, K[l])) Dhc = fminf(Dhi, ls[i + l]);
-            float xk = xq[l], Plo = __fsub_rn(xq[l], K[l]);
+            float xk = xq[l], Rp = __fsub_rn(xq[l], K[l]);
 #pragma unroll
             for (int k = 0; k < NC; ++k)
-                acc = cand_step<ARITH>(k == 0, xk, Plo, K[l], Dlo, Dhc, v[l][k], acc);
+                acc = cand_step<ARITH>(k == 0, xk, Rp, K[l], Dlo, Dhc, v[l][k], acc);
         }
     }
     for (; i <= i_hi; ++i, u = __fadd_rn(u, 1.0f), lbase += npix) {
@@ -511,10 +514,10 @@ __device__ __forceinline__ float fwd_grouped(const __amdgpu_buffer_rsrc_t rs, co
         } else {
             load_run<NC>(rs, lbase + q, npix * npix, v);
         }
-        float xk = x, Plo = __fsub_rn(x, K);
+        float xk = x, Rp = __fsub_rn(x, K);
 #pragma unroll
         for (int k = 0; k < NC; ++k)
-            acc = cand_step<ARITH>(k == 0, xk, Plo, K, Dlo, Dhc, v[k], acc);
+            acc = cand_step<ARITH>(k == 0, xk, Rp, K, Dlo, Dhc, v[k], acc);
     }
     return acc;
 }
@@ -624,10 +627,10 @@ __device__ __forceinline__ float fwd_staged(const DdView& g, const float* __rest
                 // clipping every weight's right end to L* is the exact rule; it changes
                 // only the weights that end at the line's dropped segment
                 const float Dhc = fminf(Dhi, sL[l]);
-                float xk = x, Plo = __fsub_rn(x, K);
+                float xk = x, Rp = __fsub_rn(x, K);
 #pragma unroll
                 for (int k = 0; k < NC; ++k)
-                    acc = cand_step<ARITH>(k == 0, xk, Plo, K, Dlo, Dhc, wp[k], acc);
+                    acc = cand_step<ARITH>(k == 0, xk, Rp, K, Dlo, Dhc, wp[k], acc);
             }
             u = __fadd_rn(u, 1.0f);
         }
@@ -793,13 +796,13 @@ __device__ __forceinline__ void dd_fwd_block(const DdView* __restrict__ geom,
                     const int q = (int)x + nh;
                     float Dhc = Dhi;
                     if (last_bin || Dhi > __fsub_rn(xNm1, K)) Dhc = fminf(Dhi, ls[i]);
-                    float xk = x, Plo = __fsub_rn(x, K);
+                    float xk = x, Rp = __fsub_rn(x, K);
                     for (int k0 = 0; k0 < nc; k0 += 4) {
                         float v[4] = {1.0f, 1.0f, 1.0f, 1.0f};
                         if (!ONES) load_run<4>(rs, lbase + q + k0, npix * npix, v);
 #pragma unroll
                         for (int k = 0; k < 4; ++k)
-                            acc = cand_step<ARITH>(k0 + k == 0, xk, Plo, K, Dlo, Dhc, v[k], acc);
+                            acc = cand_step<ARITH>(k0 + k == 0, xk, Rp, K, Dlo, Dhc, v[k], acc);
                     }
                 }
             }

@@ -13,6 +13,8 @@
 //   A (16 x 32, rows i, K k):  lane l holds A[l % 16][8 * (l / 16) + e], e = 0..7
 //   B (32 x 16, K k, cols j):  lane l holds B[8 * (l / 16) + e][l % 16]
 //   D (16 x 16):               lane l holds D[4 * (l / 16) + r][l % 16], r = 0..3
+#include <algorithm>
+
 #include "itomo_internal.h"
 
 namespace itomo {
@@ -168,6 +170,121 @@ conv3x3_kernel(const bf16_t* __restrict__ X, const bf16_t* __restrict__ Wt,
         }
 }
 
+// The same convolution for Cin % 32 == 0, tiled through LDS.  Block = 4 waves = a tile of
+// (4 WM) x 16 pixels of one image x 16 WN output channels; wave w computes tile rows
+// w WM .. w WM + WM - 1 (one 16-pixel MFMA M-tile each) for all 16 WN channels.  Input
+// channels are taken 32 at a time: the chunk's (4 WM + 2) x 18 halo of input pixels and
+// its 16 WN x 9 x 32 weights are staged in LDS (every global load of a chunk is issued
+// before any is stored), then the 9 taps run from LDS: per tap a wave reads WM A- and WN
+// B-fragments (16 bytes per lane) for WM x WN MFMAs.  An LDS row (one staged pixel, or one
+// output channel and tap of the weights) holds 32 channels padded to 40 (80 bytes), so
+// the 16-byte reads of 16 consecutive lanes spread over the banks.  Pixels outside the
+// image read zeros (the padding of the convolution); outputs outside it are not stored.
+constexpr int kConvChunk = 32;   // input channels per staged chunk
+constexpr int kConvRow = 40;     // bf16 per LDS row (32 + 8 padding)
+
+template <int WM, int WN>
+__global__ void __launch_bounds__(256)
+conv3x3_tiled_kernel(const bf16_t* __restrict__ X, const bf16_t* __restrict__ Wt,
+                     const float* __restrict__ bias, const bf16_t* __restrict__ mask,
+                     bf16_t* __restrict__ Y, int H, int W, int Cin, int Cout, int relu,
+                     int co_blocks)
+{
+    constexpr int TH = 4 * WM, HY = TH + 2, HX = 18, NB = 16 * WN;
+    constexpr int NXP = HY * HX * 4;       // 16-byte pieces of a staged halo chunk
+    constexpr int NWP = NB * 9 * 4;        // 16-byte pieces of a staged weight chunk
+    constexpr int PX = (NXP + 255) / 256, PW = (NWP + 255) / 256;
+    __shared__ __attribute__((aligned(16))) bf16_t sX[HY * HX * kConvRow];
+    __shared__ __attribute__((aligned(16))) bf16_t sW[NB * 9 * kConvRow];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int li = lane & 15, lg = lane >> 4;
+    const int n = blockIdx.z / co_blocks, cob = blockIdx.z - n * co_blocks;
+    const int y0 = blockIdx.y * TH, x0 = blockIdx.x * 16, co0 = cob * NB;
+    const int64_t img = (int64_t)n * H * W;
+
+    f32x4 acc[WM][WN];
+#pragma unroll
+    for (int t = 0; t < WM; ++t)
+#pragma unroll
+        for (int j = 0; j < WN; ++j) acc[t][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+    for (int c0 = 0; c0 < Cin; c0 += kConvChunk) {
+        uint4 vx[PX], vw[PW];
+#pragma unroll
+        for (int r = 0; r < PX; ++r) {
+            const int k = tid + r * 256;
+            vx[r] = make_uint4(0, 0, 0, 0);
+            if (k < NXP) {
+                const int pix = k >> 2, q = k & 3;
+                const int hy = pix / HX, hx = pix - hy * HX;
+                const int y = y0 - 1 + hy, x = x0 - 1 + hx;
+                if (y >= 0 && y < H && x >= 0 && x < W)
+                    vx[r] = *(const uint4*)(X + (img + (int64_t)y * W + x) * Cin + c0 + 8 * q);
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < PW; ++r) {
+            const int k = tid + r * 256;
+            vw[r] = make_uint4(0, 0, 0, 0);
+            if (k < NWP) {
+                const int row = k >> 2, q = k & 3;      // row = local channel * 9 + tap
+                const int co = co0 + row / 9;
+                if (co < Cout)
+                    vw[r] = *(const uint4*)(Wt + ((int64_t)co0 * 9 + row) * Cin + c0 + 8 * q);
+            }
+        }
+        __syncthreads();   // the previous chunk is consumed
+#pragma unroll
+        for (int r = 0; r < PX; ++r) {
+            const int k = tid + r * 256;
+            if (k < NXP) *(uint4*)&sX[(k >> 2) * kConvRow + 8 * (k & 3)] = vx[r];
+        }
+#pragma unroll
+        for (int r = 0; r < PW; ++r) {
+            const int k = tid + r * 256;
+            if (k < NWP) *(uint4*)&sW[(k >> 2) * kConvRow + 8 * (k & 3)] = vw[r];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int tap = 0; tap < 9; ++tap) {
+            const int ty = tap / 3, tx = tap - 3 * (tap / 3);
+            bf16x8 a[WM], b[WN];
+#pragma unroll
+            for (int t = 0; t < WM; ++t)
+                a[t] = load8(&sX[((wave * WM + t + ty) * HX + li + tx) * kConvRow + 8 * lg]);
+#pragma unroll
+            for (int j = 0; j < WN; ++j)
+                b[j] = load8(&sW[((j * 16 + li) * 9 + tap) * kConvRow + 8 * lg]);
+#pragma unroll
+            for (int t = 0; t < WM; ++t)
+#pragma unroll
+                for (int j = 0; j < WN; ++j) acc[t][j] = mfma(a[t], b[j], acc[t][j]);
+        }
+    }
+
+#pragma unroll
+    for (int j = 0; j < WN; ++j) {
+        const int co = co0 + j * 16 + li;
+        if (co >= Cout) continue;
+        const float bv = bias ? bias[co] : 0.0f;
+#pragma unroll
+        for (int t = 0; t < WM; ++t) {
+            const int y = y0 + wave * WM + t;
+            if (y >= H) continue;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int x = x0 + 4 * lg + r;
+                if (x >= W) continue;
+                float v = acc[t][j][r] + bv;
+                if (relu) v = fmaxf(v, 0.0f);
+                const int64_t o = (img + (int64_t)y * W + x) * Cout + co;
+                if (mask && !(bf2f(mask[o]) > 0.0f)) v = 0.0f;
+                Y[o] = f2bf(v);
+            }
+        }
+    }
+}
+
 // Wf[ci][tap][co] = Wt[co][8 - tap][ci]: the weights of the transposed convolution
 // (gradient with respect to the input of a 3x3, pad-1 convolution).
 __global__ void flip_kernel(const bf16_t* __restrict__ Wt, bf16_t* __restrict__ Wf, int Cin,
@@ -275,6 +392,142 @@ wgrad_kernel(const bf16_t* __restrict__ X, const bf16_t* __restrict__ dY,
                 if (co < Cout) atomicAdd(&dW[(int64_t)co * ncol + col], acc[i][j][r]);
             }
         }
+}
+
+// The weight gradient for Cin % 32 == 0 and Cout % 32 == 0, tiled through LDS.  GEMM
+// M = output channels (a block's 16 WM), N = 9 taps x 32 input channels (one chunk),
+// K = pixels in tiles of 8 x 16 (128).  The MFMA wants 8 consecutive K values per lane,
+// i.e. 8 pixels of one channel, while activations are stored channels-fastest, so both
+// operands are transposed while they are staged: dY of the tile as sD[channel][pixel],
+// and the tile's 10 x 18 input halo three times, shifted by the tap's column offset, as
+// sX[shift][channel][halo row][16 columns], so that every fragment read is one aligned
+// 16-byte LDS read.  Wave (wr, wc) computes channels wr * 8 WM .. (WM tiles of 16) x
+// taps 0..8 of input channels 16 wc .. 16 wc + 15.  Blocks take pixel tiles
+// blockIdx.z, blockIdx.z + gridDim.z, ... and add their sums to dW atomically.
+constexpr int kWgTileRows = 8;
+constexpr int kWgDRow = 136;              // bf16 per sD row: 128 pixels + 8 padding
+constexpr int kWgXRow = 24;               // bf16 per sX row: 16 columns + 8 padding
+constexpr int kWgXChan = 10 * kWgXRow + 8;  // bf16 per channel of one shifted copy
+
+template <int WM>
+__global__ void __launch_bounds__(256)
+wgrad_tiled_kernel(const bf16_t* __restrict__ X, const bf16_t* __restrict__ dY,
+                   float* __restrict__ dW, int N, int H, int W, int Cin, int Cout,
+                   int ci_chunks)
+{
+    constexpr int MB = 32 * WM;            // output channels per block (2 waves x WM tiles)
+    constexpr int ND = 128 * MB / 8;       // 16-byte pieces of the staged dY tile
+    constexpr int NX = 10 * 18 * 4;        // 16-byte pieces of the staged halo chunk
+    constexpr int PD = (ND + 255) / 256, PXP = (NX + 255) / 256;
+    __shared__ __attribute__((aligned(16))) bf16_t sD[MB * kWgDRow];
+    __shared__ __attribute__((aligned(16))) bf16_t sX[3 * 32 * kWgXChan];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int li = lane & 15, lg = lane >> 4;
+    const int wr = wave >> 1, wc = wave & 1;
+    const int cob = blockIdx.x / ci_chunks, cic = blockIdx.x - cob * ci_chunks;
+    const int co0 = cob * MB, c0 = cic * 32;
+    const int tiles_x = (W + 15) / 16, tiles_y = (H + kWgTileRows - 1) / kWgTileRows;
+    const int64_t ntiles = (int64_t)N * tiles_y * tiles_x;
+
+    f32x4 acc[WM][9];
+#pragma unroll
+    for (int i = 0; i < WM; ++i)
+#pragma unroll
+        for (int t = 0; t < 9; ++t) acc[i][t] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+    for (int64_t tile = blockIdx.z; tile < ntiles; tile += gridDim.z) {
+        const int n = (int)(tile / ((int64_t)tiles_y * tiles_x));
+        const int rem = (int)(tile - (int64_t)n * tiles_y * tiles_x);
+        const int ty0 = (rem / tiles_x) * kWgTileRows, tx0 = (rem % tiles_x) * 16;
+        const int64_t img = (int64_t)n * H * W;
+        // global -> registers: dY pieces (pixel k % 128, channels 8 (k / 128) ..) and
+        // halo pieces (halo pixel k / 4, channels 8 (k % 4) ..); zeros outside
+        uint4 vd[PD], vx[PXP];
+#pragma unroll
+        for (int r = 0; r < PD; ++r) {
+            const int k = tid + r * 256;
+            vd[r] = make_uint4(0, 0, 0, 0);
+            if (k < ND) {
+                const int px = k & 127, cg = k >> 7;
+                const int y = ty0 + (px >> 4), x = tx0 + (px & 15);
+                if (y < H && x < W)
+                    vd[r] = *(const uint4*)(dY + (img + (int64_t)y * W + x) * Cout + co0 + 8 * cg);
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < PXP; ++r) {
+            const int k = tid + r * 256;
+            vx[r] = make_uint4(0, 0, 0, 0);
+            if (k < NX) {
+                const int hp = k >> 2, q = k & 3;
+                const int hy = hp / 18, hx = hp - hy * 18;
+                const int y = ty0 - 1 + hy, x = tx0 - 1 + hx;
+                if (y >= 0 && y < H && x >= 0 && x < W)
+                    vx[r] = *(const uint4*)(X + (img + (int64_t)y * W + x) * Cin + c0 + 8 * q);
+            }
+        }
+        __syncthreads();   // the previous tile is consumed
+#pragma unroll
+        for (int r = 0; r < PD; ++r) {
+            const int k = tid + r * 256;
+            if (k < ND) {
+                const int px = k & 127, cg = k >> 7;
+                const bf16_t* e = (const bf16_t*)&vd[r];
+#pragma unroll
+                for (int j = 0; j < 8; ++j) sD[(8 * cg + j) * kWgDRow + px] = e[j];
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < PXP; ++r) {
+            const int k = tid + r * 256;
+            if (k < NX) {
+                const int hp = k >> 2, q = k & 3;
+                const int hy = hp / 18, hx = hp - hy * 18;
+                const bf16_t* e = (const bf16_t*)&vx[r];
+#pragma unroll
+                for (int sh = 0; sh < 3; ++sh) {
+                    const int col = hx - sh;   // column of the copy shifted by sh
+                    if (col >= 0 && col < 16) {
+#pragma unroll
+                        for (int j = 0; j < 8; ++j)
+                            sX[(sh * 32 + 8 * q + j) * kWgXChan + hy * kWgXRow + col] = e[j];
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        // K steps of 32 pixels: tile rows 2 s, 2 s + 1; lane group lg holds row
+        // 2 s + (lg >> 1), columns 8 (lg & 1) .. + 7
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            const int row = 2 * s + (lg >> 1), col = 8 * (lg & 1);
+            bf16x8 a[WM];
+#pragma unroll
+            for (int i = 0; i < WM; ++i)
+                a[i] = load8(&sD[((wr * WM + i) * 16 + li) * kWgDRow + row * 16 + col]);
+#pragma unroll
+            for (int t = 0; t < 9; ++t) {
+                const int dy = t / 3, sh = t - 3 * (t / 3);
+                const bf16x8 b =
+                    load8(&sX[(sh * 32 + 16 * wc + li) * kWgXChan + (row + dy) * kWgXRow + col]);
+#pragma unroll
+                for (int i = 0; i < WM; ++i) acc[i][t] = mfma(a[i], b, acc[i][t]);
+            }
+        }
+    }
+    // acc[i][t] lane: channel co0 + (wr WM + i) 16 + 4 lg + r, tap t, input channel
+    // c0 + 16 wc + li
+    const int ncol = 9 * Cin;
+#pragma unroll
+    for (int i = 0; i < WM; ++i)
+#pragma unroll
+        for (int t = 0; t < 9; ++t)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int co = co0 + (wr * WM + i) * 16 + 4 * lg + r;
+                if (co < Cout)
+                    atomicAdd(&dW[(int64_t)co * ncol + t * Cin + c0 + 16 * wc + li], acc[i][t][r]);
+            }
 }
 
 // db[c] += sum over the block's pixels of dY[p][c]; thread c of a block of C threads.
@@ -527,8 +780,25 @@ int itomo_unet_conv3x3(const void* x, int n, int h, int w, int cin, const void* 
     if (P > ((int64_t)1 << 31) / 8 || (int64_t)9 * cin > (1 << 20))
         return set_error(ITOMO_EINVAL, "unet_conv3x3: too large");
     const bool vec = cin % 8 == 0 && ((uintptr_t)x % 16) == 0 && ((uintptr_t)wt % 16) == 0;
-    dim3 grid((unsigned)((P + 127) / 128), (unsigned)((cout + 31) / 32));
     hipStream_t st = (hipStream_t)stream;
+    if (vec && cin % kConvChunk == 0 && cout >= 32) {
+        // 16 x 16-pixel tiles; 64 output channels per block, or 32 where that is all
+        const bool wide = cout > 32;
+        const int nb = wide ? 64 : 32;
+        const int co_blocks = (cout + nb - 1) / nb;
+        if ((int64_t)n * co_blocks > 65535) return set_error(ITOMO_EINVAL, "unet_conv3x3: too large");
+        dim3 g((unsigned)((w + 15) / 16), (unsigned)((h + 15) / 16), (unsigned)(n * co_blocks));
+        if (wide)
+            conv3x3_tiled_kernel<4, 4><<<g, 256, 0, st>>>((const bf16_t*)x, (const bf16_t*)wt, bias,
+                                                          (const bf16_t*)mask, (bf16_t*)y, h, w,
+                                                          cin, cout, relu, co_blocks);
+        else
+            conv3x3_tiled_kernel<4, 2><<<g, 256, 0, st>>>((const bf16_t*)x, (const bf16_t*)wt, bias,
+                                                          (const bf16_t*)mask, (bf16_t*)y, h, w,
+                                                          cin, cout, relu, co_blocks);
+        return check_hip(hipGetLastError(), "unet_conv3x3 launch");
+    }
+    dim3 grid((unsigned)((P + 127) / 128), (unsigned)((cout + 31) / 32));
     if (vec)
         conv3x3_kernel<true><<<grid, 256, 0, st>>>((const bf16_t*)x, (const bf16_t*)wt, bias,
                                                    (const bf16_t*)mask, (bf16_t*)y, n, h, w,
@@ -556,6 +826,22 @@ int itomo_unet_wgrad(const void* x, const void* dy, int n, int h, int w, int cin
         return set_error(ITOMO_EINVAL, "unet_wgrad: bad argument");
     const int64_t P = (int64_t)n * h * w;
     hipStream_t st = (hipStream_t)stream;
+    if (cin % 32 == 0 && cout % 32 == 0 && ((uintptr_t)x % 16) == 0 && ((uintptr_t)dy % 16) == 0) {
+        // 64 output channels per block where they divide, else 32; about 2048 blocks
+        const bool wide = cout % 64 == 0;
+        const int mb = wide ? 64 : 32;
+        const int ci_chunks = cin / 32;
+        const int gxw = (cout / mb) * ci_chunks;
+        const int64_t ntiles = (int64_t)n * ((h + 7) / 8) * ((w + 15) / 16);
+        int64_t gz = 2048 / gxw;
+        gz = std::max<int64_t>(1, std::min<int64_t>(std::min<int64_t>(gz, ntiles), 65535));
+        if (wide)
+            wgrad_tiled_kernel<2><<<dim3((unsigned)gxw, 1, (unsigned)gz), 256, 0, st>>>(
+                (const bf16_t*)x, (const bf16_t*)dy, dw, n, h, w, cin, cout, ci_chunks);
+        else
+            wgrad_tiled_kernel<1><<<dim3((unsigned)gxw, 1, (unsigned)gz), 256, 0, st>>>(
+                (const bf16_t*)x, (const bf16_t*)dy, dw, n, h, w, cin, cout, ci_chunks);
+    } else {
     const int ncol = 9 * cin;
     const unsigned gx = (unsigned)((ncol + 63) / 64), gy = (unsigned)((cout + 63) / 64);
     // split the pixels so that the grid holds ~2048 blocks, in multiples of 32 pixels
@@ -566,6 +852,7 @@ int itomo_unet_wgrad(const void* x, const void* dy, int n, int h, int w, int cin
     nz = (P + chunk - 1) / chunk;
     wgrad_kernel<<<dim3(gx, gy, (unsigned)nz), 256, 0, st>>>((const bf16_t*)x, (const bf16_t*)dy,
                                                              dw, n, h, w, cin, cout, chunk);
+    }
     int rc = check_hip(hipGetLastError(), "unet_wgrad launch");
     if (rc || !db) return rc;
     const int64_t bchunk = 1024;

@@ -59,7 +59,9 @@ def ulp_diff(a_bf16, b_bf16):
 @pytest.mark.parametrize("n,h,w,cin,cout,relu", [(2, 16, 16, 32, 48, True),
                                                   (1, 10, 14, 1, 32, True),
                                                   (3, 8, 8, 96, 32, False),
-                                                  (1, 12, 12, 64, 64, True)])
+                                                  (1, 12, 12, 64, 64, True),
+                                                  (2, 37, 21, 64, 160, True),
+                                                  (1, 40, 48, 384, 128, False)])
 def test_conv3x3_matches_fp64(n, h, w, cin, cout, relu):
     x = bf16_rand((n, h, w, cin), 1)
     wt = bf16_rand((cout, 9, cin), 2, scale=(2.0 / (9 * cin)) ** 0.5)
@@ -80,12 +82,24 @@ def test_conv3x3_matches_fp64(n, h, w, cin, cout, relu):
         ref16 = ref.to(torch.float32).to(torch.bfloat16)
         got = y.cpu()
         d = ulp_diff(got, ref16)
-        assert int(d.max()) <= 1
-        assert float((d > 0).double().mean()) < 0.01
+        if 9 * cin <= 1152:
+            assert int(d.max()) <= 1
+            assert float((d > 0).double().mean()) < 0.01
+        else:
+            # long sums: FP32 accumulation in another order than FP64's shows on outputs
+            # near zero, whose BF16 ulp is tiny; bound it against the output scale
+            rms = float(ref.pow(2).mean().sqrt())
+            big = ref.abs() >= 0.05 * rms
+            assert int(d[big].max()) <= 1
+            # everywhere: one BF16 ulp of the value plus 2e-3 of the output scale
+            ulp = torch.exp2(torch.floor(torch.log2(ref.abs().clamp_min(1e-30))) - 7)
+            assert bool(((got.double() - ref).abs() <= ulp + 2e-3 * rms).all())
+            assert float((d > 0).double().mean()) < 0.02
 
 
 @pytest.mark.parametrize("n,h,w,cin,cout", [(2, 16, 16, 32, 64), (1, 12, 10, 1, 32),
-                                             (2, 8, 8, 96, 32), (1, 32, 32, 64, 128)])
+                                             (2, 8, 8, 96, 32), (1, 32, 32, 64, 128),
+                                             (1, 37, 21, 64, 96), (2, 24, 40, 128, 256)])
 def test_wgrad_matches_fp64(n, h, w, cin, cout):
     x = bf16_rand((n, h, w, cin), 5)
     dy = bf16_rand((n, h, w, cout), 6)
