@@ -170,13 +170,61 @@ conv3x3_kernel(const bf16_t* __restrict__ X, const bf16_t* __restrict__ Wt,
         }
 }
 
+// The same convolution for one input channel (the network's first layer, K = 9): one
+// thread per pixel, FP32 sums, eight output channels per 16-byte store; the weights sit in
+// LDS as floats.  Cout % 8 == 0, at most 64.
+__global__ void __launch_bounds__(256)
+conv3x3_c1_kernel(const bf16_t* __restrict__ X, const bf16_t* __restrict__ Wt,
+                  const float* __restrict__ bias, const bf16_t* __restrict__ mask,
+                  bf16_t* __restrict__ Y, int N, int H, int W, int Cout, int relu)
+{
+    __shared__ float sw[64 * 9];
+    __shared__ float sb[64];
+    for (int k = threadIdx.x; k < Cout * 9; k += blockDim.x) sw[k] = bf2f(Wt[k]);
+    for (int k = threadIdx.x; k < Cout; k += blockDim.x) sb[k] = bias ? bias[k] : 0.0f;
+    __syncthreads();
+    const int64_t P = (int64_t)N * H * W;
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P) return;
+    const int64_t hw = (int64_t)H * W;
+    const int64_t n = p / hw;
+    const int r = (int)(p - n * hw);
+    const int y = r / W, x = r - (r / W) * W;
+    float v[9];
+#pragma unroll
+    for (int t = 0; t < 9; ++t) {
+        const int yy = y + t / 3 - 1, xx = x + t % 3 - 1;
+        v[t] = (yy >= 0 && yy < H && xx >= 0 && xx < W) ? bf2f(X[n * hw + (int64_t)yy * W + xx])
+                                                        : 0.0f;
+    }
+    for (int c0 = 0; c0 < Cout; c0 += 8) {
+        Frag m;
+        if (mask) m.u = *(const uint4*)(mask + p * Cout + c0);
+        const bf16_t* mh = (const bf16_t*)&m.u;
+        unsigned short o[8];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            float a = sb[c0 + e];
+#pragma unroll
+            for (int t = 0; t < 9; ++t) a = fmaf(v[t], sw[(c0 + e) * 9 + t], a);
+            if (relu) a = fmaxf(a, 0.0f);
+            if (mask && !(bf2f(mh[e]) > 0.0f)) a = 0.0f;
+            o[e] = f2bf(a);
+        }
+        *(uint4*)(Y + p * Cout + c0) =
+            make_uint4(o[0] | ((unsigned)o[1] << 16), o[2] | ((unsigned)o[3] << 16),
+                       o[4] | ((unsigned)o[5] << 16), o[6] | ((unsigned)o[7] << 16));
+    }
+}
+
 // The same convolution for Cin % 32 == 0, tiled through LDS.  Block = 4 waves = a tile of
 // (4 WM) x 16 pixels of one image x 16 WN output channels; wave w computes tile rows
 // w WM .. w WM + WM - 1 (one 16-pixel MFMA M-tile each) for all 16 WN channels.  Input
 // channels are taken 32 at a time: the chunk's (4 WM + 2) x 18 halo of input pixels and
 // its 16 WN x 9 x 32 weights are staged in LDS (every global load of a chunk is issued
-// before any is stored), then the 9 taps run from LDS: per tap a wave reads WM A- and WN
-// B-fragments (16 bytes per lane) for WM x WN MFMAs.  An LDS row (one staged pixel, or one
+// before any is stored), then the 9 taps run from LDS in order: a wave reads its WM + 2
+// halo rows at the three column offsets once per chunk (A-fragments, 16 bytes per lane;
+// each serves up to three taps), and per tap WN B-fragments, for WM x WN MFMAs per tap.  An LDS row (one staged pixel, or one
 // output channel and tap of the weights) holds 32 channels padded to 40 (80 bytes), so
 // the 16-byte reads of 16 consecutive lanes spread over the banks.  Pixels outside the
 // image read zeros (the padding of the convolution); outputs outside it are not stored.
@@ -245,20 +293,25 @@ conv3x3_tiled_kernel(const bf16_t* __restrict__ X, const bf16_t* __restrict__ Wt
             if (k < NWP) *(uint4*)&sW[(k >> 2) * kConvRow + 8 * (k & 3)] = vw[r];
         }
         __syncthreads();
+        // the wave's (WM + 2) halo rows at the three column offsets serve all nine taps:
+        // read once, then the taps in order 0..8
+        bf16x8 a[3][WM + 2];
+#pragma unroll
+        for (int tx = 0; tx < 3; ++tx)
+#pragma unroll
+            for (int t = 0; t < WM + 2; ++t)
+                a[tx][t] = load8(&sX[((wave * WM + t) * HX + li + tx) * kConvRow + 8 * lg]);
 #pragma unroll
         for (int tap = 0; tap < 9; ++tap) {
             const int ty = tap / 3, tx = tap - 3 * (tap / 3);
-            bf16x8 a[WM], b[WN];
-#pragma unroll
-            for (int t = 0; t < WM; ++t)
-                a[t] = load8(&sX[((wave * WM + t + ty) * HX + li + tx) * kConvRow + 8 * lg]);
+            bf16x8 b[WN];
 #pragma unroll
             for (int j = 0; j < WN; ++j)
                 b[j] = load8(&sW[((j * 16 + li) * 9 + tap) * kConvRow + 8 * lg]);
 #pragma unroll
             for (int t = 0; t < WM; ++t)
 #pragma unroll
-                for (int j = 0; j < WN; ++j) acc[t][j] = mfma(a[t], b[j], acc[t][j]);
+                for (int j = 0; j < WN; ++j) acc[t][j] = mfma(a[tx][t + ty], b[j], acc[t][j]);
         }
     }
 
@@ -401,9 +454,10 @@ wgrad_kernel(const bf16_t* __restrict__ X, const bf16_t* __restrict__ dY,
 // operands are transposed while they are staged: dY of the tile as sD[channel][pixel],
 // and the tile's 10 x 18 input halo three times, shifted by the tap's column offset, as
 // sX[shift][channel][halo row][16 columns], so that every fragment read is one aligned
-// 16-byte LDS read.  Wave (wr, wc) computes channels wr * 8 WM .. (WM tiles of 16) x
-// taps 0..8 of input channels 16 wc .. 16 wc + 15.  Blocks take pixel tiles
-// blockIdx.z, blockIdx.z + gridDim.z, ... and add their sums to dW atomically.
+// 16-byte LDS read.  Wave (wr, wc) computes the WM 16-channel tiles from channel 16 WM wr,
+// for taps 0..8 and input channels 16 wc .. 16 wc + 15.  Blocks take pixel tiles
+// blockIdx.z, blockIdx.z + gridDim.z, ... and add their sums to dW atomically; the bias
+// gradient comes out of the same pass (see bias_here).
 constexpr int kWgTileRows = 8;
 constexpr int kWgDRow = 136;              // bf16 per sD row: 128 pixels + 8 padding
 constexpr int kWgXRow = 24;               // bf16 per sX row: 16 columns + 8 padding
@@ -412,8 +466,8 @@ constexpr int kWgXChan = 10 * kWgXRow + 8;  // bf16 per channel of one shifted c
 template <int WM>
 __global__ void __launch_bounds__(256)
 wgrad_tiled_kernel(const bf16_t* __restrict__ X, const bf16_t* __restrict__ dY,
-                   float* __restrict__ dW, int N, int H, int W, int Cin, int Cout,
-                   int ci_chunks)
+                   float* __restrict__ dW, float* __restrict__ db, int N, int H, int W,
+                   int Cin, int Cout, int ci_chunks)
 {
     constexpr int MB = 32 * WM;            // output channels per block (2 waves x WM tiles)
     constexpr int ND = 128 * MB / 8;       // 16-byte pieces of the staged dY tile
@@ -434,6 +488,15 @@ wgrad_tiled_kernel(const bf16_t* __restrict__ X, const bf16_t* __restrict__ dY,
     for (int i = 0; i < WM; ++i)
 #pragma unroll
         for (int t = 0; t < 9; ++t) acc[i][t] = f32x4{0.f, 0.f, 0.f, 0.f};
+    // the bias gradient: the same A fragments against a B of ones (every column of D is
+    // the sum over the K pixels), in the blocks of the first input-channel chunk, by the
+    // waves of the first channel half
+    const bool bias_here = db && cic == 0 && wc == 0;
+    Frag ones;
+    ones.u = make_uint4(0x3f803f80u, 0x3f803f80u, 0x3f803f80u, 0x3f803f80u);
+    f32x4 accb[WM];
+#pragma unroll
+    for (int i = 0; i < WM; ++i) accb[i] = f32x4{0.f, 0.f, 0.f, 0.f};
 
     for (int64_t tile = blockIdx.z; tile < ntiles; tile += gridDim.z) {
         const int n = (int)(tile / ((int64_t)tiles_y * tiles_x));
@@ -505,6 +568,10 @@ wgrad_tiled_kernel(const bf16_t* __restrict__ X, const bf16_t* __restrict__ dY,
 #pragma unroll
             for (int i = 0; i < WM; ++i)
                 a[i] = load8(&sD[((wr * WM + i) * 16 + li) * kWgDRow + row * 16 + col]);
+            if (bias_here) {
+#pragma unroll
+                for (int i = 0; i < WM; ++i) accb[i] = mfma(a[i], ones.v, accb[i]);
+            }
 #pragma unroll
             for (int t = 0; t < 9; ++t) {
                 const int dy = t / 3, sh = t - 3 * (t / 3);
@@ -528,6 +595,78 @@ wgrad_tiled_kernel(const bf16_t* __restrict__ X, const bf16_t* __restrict__ dY,
                 if (co < Cout)
                     atomicAdd(&dW[(int64_t)co * ncol + t * Cin + c0 + 16 * wc + li], acc[i][t][r]);
             }
+    if (bias_here && li == 0) {
+#pragma unroll
+        for (int i = 0; i < WM; ++i)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int co = co0 + (wr * WM + i) * 16 + 4 * lg + r;
+                if (co < Cout) atomicAdd(&db[co], accb[i][r]);
+            }
+    }
+}
+
+// The weight and bias gradients of the one-input-channel layer: dW[co][tap] and db[co]
+// over the pixels [blockIdx.x * chunk, +chunk).  Thread (pl, cg) of a block (32 x 8)
+// takes output channels 4 cg .. 4 cg + 3 and the pixels pl, pl + 32, ... of the chunk,
+// walking (image, row, column) incrementally; FP32 sums, added over the 32 pixel lanes in
+// LDS and once atomically.  Cout == 32.
+__global__ void __launch_bounds__(256)
+wgrad_c1_kernel(const bf16_t* __restrict__ X, const bf16_t* __restrict__ dY,
+                float* __restrict__ dW, float* __restrict__ db, int N, int H, int W,
+                int64_t chunk)
+{
+    __shared__ float part[32][32 * 10 + 1];
+    const int pl = threadIdx.x & 31, cg = threadIdx.x >> 5;
+    const int64_t P = (int64_t)N * H * W, hw = (int64_t)H * W;
+    const int64_t pb = (int64_t)blockIdx.x * chunk, pe = min(P, pb + chunk);
+    float a[4][10];
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+#pragma unroll
+        for (int t = 0; t < 10; ++t) a[c][t] = 0.0f;
+    int64_t p = pb + pl;
+    int64_t n = p / hw;
+    int rem = (int)(p - n * hw);
+    int y = rem / W, x = rem - (rem / W) * W;
+    for (; p < pe; p += 32) {
+        const uint2 gv = *(const uint2*)(dY + p * 32 + 4 * cg);
+        const float g[4] = {__uint_as_float(gv.x << 16), __uint_as_float(gv.x & 0xffff0000u),
+                            __uint_as_float(gv.y << 16), __uint_as_float(gv.y & 0xffff0000u)};
+        const bf16_t* xi = X + n * hw;
+#pragma unroll
+        for (int t = 0; t < 9; ++t) {
+            const int yy = y + t / 3 - 1, xx = x + t % 3 - 1;
+            const float v = (yy >= 0 && yy < H && xx >= 0 && xx < W)
+                                ? bf2f(xi[(int64_t)yy * W + xx]) : 0.0f;
+#pragma unroll
+            for (int c = 0; c < 4; ++c) a[c][t] = fmaf(g[c], v, a[c][t]);
+        }
+#pragma unroll
+        for (int c = 0; c < 4; ++c) a[c][9] += g[c];
+        x += 32;                          // the next pixel of this lane
+        while (x >= W) {
+            x -= W;
+            if (++y == H) {
+                y = 0;
+                ++n;
+            }
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+#pragma unroll
+        for (int t = 0; t < 10; ++t) part[pl][(4 * cg + c) * 10 + t] = a[c][t];
+    __syncthreads();
+    for (int k = threadIdx.x; k < 32 * 10; k += blockDim.x) {
+        float sum = 0.0f;
+        for (int q = 0; q < 32; ++q) sum += part[q][k];
+        const int c = k / 10, t = k - c * 10;
+        if (t < 9)
+            atomicAdd(&dW[c * 9 + t], sum);
+        else if (db)
+            atomicAdd(&db[c], sum);
+    }
 }
 
 // db[c] += sum over the block's pixels of dY[p][c]; thread c of a block of C threads.
@@ -781,13 +920,20 @@ int itomo_unet_conv3x3(const void* x, int n, int h, int w, int cin, const void* 
         return set_error(ITOMO_EINVAL, "unet_conv3x3: too large");
     const bool vec = cin % 8 == 0 && ((uintptr_t)x % 16) == 0 && ((uintptr_t)wt % 16) == 0;
     hipStream_t st = (hipStream_t)stream;
+    if (cin == 1 && cout % 8 == 0 && cout <= 64 && ((uintptr_t)y % 16) == 0 &&
+        ((uintptr_t)mask % 16) == 0) {
+        conv3x3_c1_kernel<<<(unsigned)((P + 255) / 256), 256, 0, st>>>(
+            (const bf16_t*)x, (const bf16_t*)wt, bias, (const bf16_t*)mask, (bf16_t*)y, n, h, w,
+            cout, relu);
+        return check_hip(hipGetLastError(), "unet_conv3x3 launch");
+    }
     if (vec && cin % kConvChunk == 0 && cout >= 32) {
         // 16 x 16-pixel tiles; 64 output channels per block, or 32 where that is all
         const bool wide = cout > 32;
-        const int nb = wide ? 64 : 32;
+        const int nb = wide ? 64 : 32, th = 16;
         const int co_blocks = (cout + nb - 1) / nb;
         if ((int64_t)n * co_blocks > 65535) return set_error(ITOMO_EINVAL, "unet_conv3x3: too large");
-        dim3 g((unsigned)((w + 15) / 16), (unsigned)((h + 15) / 16), (unsigned)(n * co_blocks));
+        dim3 g((unsigned)((w + 15) / 16), (unsigned)((h + th - 1) / th), (unsigned)(n * co_blocks));
         if (wide)
             conv3x3_tiled_kernel<4, 4><<<g, 256, 0, st>>>((const bf16_t*)x, (const bf16_t*)wt, bias,
                                                           (const bf16_t*)mask, (bf16_t*)y, h, w,
@@ -826,6 +972,13 @@ int itomo_unet_wgrad(const void* x, const void* dy, int n, int h, int w, int cin
         return set_error(ITOMO_EINVAL, "unet_wgrad: bad argument");
     const int64_t P = (int64_t)n * h * w;
     hipStream_t st = (hipStream_t)stream;
+    if (cin == 1 && cout == 32 && ((uintptr_t)dy % 8) == 0) {
+        // about 2048 blocks, at least 256 pixels each
+        const int64_t chunk = std::max<int64_t>(256, ((P + 2047) / 2048 + 31) / 32 * 32);
+        wgrad_c1_kernel<<<(unsigned)((P + chunk - 1) / chunk), 256, 0, st>>>(
+            (const bf16_t*)x, (const bf16_t*)dy, dw, db, n, h, w, chunk);
+        return check_hip(hipGetLastError(), "unet_wgrad launch");
+    }
     if (cin % 32 == 0 && cout % 32 == 0 && ((uintptr_t)x % 16) == 0 && ((uintptr_t)dy % 16) == 0) {
         // 64 output channels per block where they divide, else 32; about 2048 blocks
         const bool wide = cout % 64 == 0;
@@ -833,14 +986,16 @@ int itomo_unet_wgrad(const void* x, const void* dy, int n, int h, int w, int cin
         const int ci_chunks = cin / 32;
         const int gxw = (cout / mb) * ci_chunks;
         const int64_t ntiles = (int64_t)n * ((h + 7) / 8) * ((w + 15) / 16);
-        int64_t gz = 2048 / gxw;
-        gz = std::max<int64_t>(1, std::min<int64_t>(std::min<int64_t>(gz, ntiles), 65535));
+        // at least 4 pixel tiles per block, so that the atomic adds stay few
+        int64_t gz = std::min<int64_t>(2048 / gxw, (ntiles + 3) / 4);
+        gz = std::max<int64_t>(1, std::min<int64_t>(gz, 65535));
         if (wide)
             wgrad_tiled_kernel<2><<<dim3((unsigned)gxw, 1, (unsigned)gz), 256, 0, st>>>(
-                (const bf16_t*)x, (const bf16_t*)dy, dw, n, h, w, cin, cout, ci_chunks);
+                (const bf16_t*)x, (const bf16_t*)dy, dw, db, n, h, w, cin, cout, ci_chunks);
         else
             wgrad_tiled_kernel<1><<<dim3((unsigned)gxw, 1, (unsigned)gz), 256, 0, st>>>(
-                (const bf16_t*)x, (const bf16_t*)dy, dw, n, h, w, cin, cout, ci_chunks);
+                (const bf16_t*)x, (const bf16_t*)dy, dw, db, n, h, w, cin, cout, ci_chunks);
+        return check_hip(hipGetLastError(), "unet_wgrad launch");
     } else {
     const int ncol = 9 * cin;
     const unsigned gx = (unsigned)((ncol + 63) / 64), gy = (unsigned)((cout + 63) / 64);
