@@ -48,6 +48,31 @@ def fbp(plan, sino, ftype="ramp"):
     return torch.flip(reco, dims=(-2, -1)).contiguous()
 
 
+_GRIDREC = {}
+
+
+def gridrec_fbp(plan, sino):
+    """Gridding (gridrec) filtered back projection of the distance-driven plan's (B, V, N)
+    sinograms: their rows are the views of angles_deg(V) in order, with the bins in the
+    gridding pair's orientation, and the rotation axis lies between the two middle bins,
+    i.e. at N/2 - 1/2 in gridrec's centre coordinate (the PSWF defaults of
+    class_projectors_grid.py:50)."""
+    from .gridrec import GridrecPlan
+    key = (plan.npix, plan.nang, torch.cuda.current_device())
+    if key not in _GRIDREC:
+        _GRIDREC[key] = GridrecPlan(plan.npix, angles_deg(plan.nang), ctr=plan.npix * 0.5 - 0.5)
+    return _GRIDREC[key].back(sino.contiguous(), filtered=True)
+
+
+def fbp_start(plan, sino, kind="dd"):
+    """The FBP start: "dd" (fbp above) or "gridrec" (gridrec_fbp)."""
+    if kind == "gridrec":
+        return gridrec_fbp(plan, sino)
+    if kind != "dd":
+        raise ValueError("FBP kind %r (have 'dd', 'gridrec')" % kind)
+    return fbp(plan, sino)
+
+
 def sirt(plan, b, iters, x0=None, pc_beta=0.0):
     """Plain SIRT (art.py:212, 314-321) on (B, V, N) sinograms, all slices in each
     launch; returns the (B, N, N) iterate."""
@@ -79,11 +104,12 @@ def _noisy(plan, x_true, i0, mu_max, gen):
 
 
 def make_pairs(plan, count, sirt_iters, seed=0, noise=(1e3, 1e5), mu_max=3.0,
-               n_ellipses=10, texture=0.15, chunk=64, input_iters=0):
+               n_ellipses=10, texture=0.15, chunk=64, input_iters=0, fbp_kind="dd",
+               verbose=False):
     """(inputs, targets, sinograms, phantoms), each (count, ...) float32 on the GPU.
-    ``noise`` = (I0_lo, I0_hi) or None for noiseless data.  The input is the FBP
-    (``input_iters`` = 0) or the plain SIRT iterate after ``input_iters`` iterations,
-    which SIRT then continues to the target."""
+    ``noise`` = (I0_lo, I0_hi) or None for noiseless data.  The input is the FBP of
+    ``fbp_kind`` (``input_iters`` = 0) or the plain SIRT iterate after ``input_iters``
+    iterations, which SIRT then continues to the target."""
     dev = device()
     gen = torch.Generator(device=dev).manual_seed(seed)
     rng = np.random.default_rng(seed)
@@ -106,10 +132,13 @@ def make_pairs(plan, count, sirt_iters, seed=0, noise=(1e3, 1e5), mu_max=3.0,
             ins.append(xi)
             outs.append(sirt(plan, b, sirt_iters - input_iters, x0=xi))
         else:
-            ins.append(fbp(plan, b))
+            ins.append(fbp_start(plan, b, fbp_kind))
             outs.append(sirt(plan, b, sirt_iters))
         sinos.append(b)
         truths.append(x_true)
+        if verbose:
+            torch.cuda.synchronize()
+            print("pairs %d/%d" % (k0 + nz, count), flush=True)
     return torch.cat(ins), torch.cat(outs), torch.cat(sinos), torch.cat(truths)
 
 
@@ -197,18 +226,18 @@ def _time_ms(fn, reps=3):
     return (time.perf_counter() - t0) * 1e3 / reps
 
 
-def compare_starts(plan, net, b, plain_iters, max_iters, input_iters=0):
+def compare_starts(plan, net, b, plain_iters, max_iters, input_iters=0, fbp_kind="dd"):
     """Time for plain SIRT from zero, from FBP and from the network's output to reach the
     residual plain SIRT from zero has after ``plain_iters`` iterations, on the (B, V, N)
-    sinograms b.  The network's input is FBP(b) (``input_iters`` = 0) or the plain SIRT
-    iterate after ``input_iters`` iterations, whose cost is then part of its time.  Wall
-    time = start-up (FBP, input iterations, network) + iterations x measured time per
-    iteration for the whole batch."""
+    sinograms b.  The network's input is FBP(b) of ``fbp_kind`` (``input_iters`` = 0) or
+    the plain SIRT iterate after ``input_iters`` iterations, whose cost is then part of
+    its time.  Wall time = start-up (FBP, input iterations, network) + iterations x
+    measured time per iteration for the whole batch."""
     nz = b.shape[0]
     x_plain = sirt(plan, b, plain_iters)
     target = residual(plan, x_plain, b) * (1.0 + 1e-9)
     zero = torch.zeros((nz, plan.npix, plan.npix), device=b.device)
-    x_fbp = fbp(plan, b)
+    x_fbp = fbp_start(plan, b, fbp_kind)
     x_in = sirt(plan, b, input_iters) if input_iters > 0 else x_fbp
     x_net = net(x_in)
     k_zero, x_z = iterations_to(plan, b, zero, target, max_iters)
@@ -218,7 +247,7 @@ def compare_starts(plan, net, b, plain_iters, max_iters, input_iters=0):
     xx = zero.clone()
     work = torch.empty_like(b)
     t_iter = _time_ms(lambda: plan.sirt_iterate(xx, b, R, C, work, pc_beta=0.0), reps=10)
-    t_fbp = _time_ms(lambda: fbp(plan, b))
+    t_fbp = _time_ms(lambda: fbp_start(plan, b, fbp_kind))
     t_net = _time_ms(lambda: net(x_in))
     t_in = t_fbp if input_iters == 0 else input_iters * t_iter
     kz, kf, kn = int(k_zero.max()), int(k_fbp.max()), int(k_net.max())
@@ -234,7 +263,7 @@ def compare_starts(plan, net, b, plain_iters, max_iters, input_iters=0):
         "ms_per_iter": t_iter, "ms_fbp": t_fbp, "ms_unet": t_net,
         "seconds_zero": kz * t_iter / 1e3,
         "seconds_fbp": (t_fbp + kf * t_iter) / 1e3,
-        "input_iters": input_iters,
+        "input_iters": input_iters, "fbp": fbp_kind,
         "seconds_unet": (t_in + t_net + kn * t_iter) / 1e3,
         "rel_l2_unet_vs_plain": rel_to_plain,
         "rel_l2_unet_vs_plain_max_slice": float(rel_slices.max()),
@@ -277,6 +306,8 @@ def main(argv=None):
     ap.add_argument("--plain-iters", type=int, default=0,
                     help="target residual: plain SIRT after this many iterations "
                          "(default: --sirt-iters)")
+    ap.add_argument("--fbp", default="dd", choices=("dd", "gridrec"),
+                    help="FBP of the network input and of the FBP start")
     ap.add_argument("--save", default="")
     args = ap.parse_args(argv)
     plan = dd_plan(args.npix, args.nang)
@@ -284,7 +315,8 @@ def main(argv=None):
     t0 = time.perf_counter()
     xin, xout, _, _ = make_pairs(plan, args.pairs, args.sirt_iters, seed=1, noise=noise,
                                  n_ellipses=args.ellipses, texture=args.texture,
-                                 chunk=args.chunk, input_iters=args.input_iters)
+                                 chunk=args.chunk, input_iters=args.input_iters,
+                                 fbp_kind=args.fbp, verbose=True)
     torch.cuda.synchronize()
     t_data = time.perf_counter() - t0
     net = UNet(seed=0)
@@ -298,7 +330,7 @@ def main(argv=None):
                                  n_ellipses=args.ellipses, texture=args.texture,
                                  chunk=args.chunk)
     res = compare_starts(plan, net, b_test, args.plain_iters or args.sirt_iters,
-                         args.max_iters, input_iters=args.input_iters)
+                         args.max_iters, input_iters=args.input_iters, fbp_kind=args.fbp)
     res.update({"pairs": args.pairs, "train_steps": args.steps, "seconds_data": t_data,
                 "seconds_train": t_train, "loss_first": float(losses[:10].mean()),
                 "loss_last": float(losses[-10:].mean()), "parameters": net.num_parameters()})
