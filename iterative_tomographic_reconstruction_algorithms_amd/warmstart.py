@@ -105,11 +105,14 @@ def _noisy(plan, x_true, i0, mu_max, gen):
 
 def make_pairs(plan, count, sirt_iters, seed=0, noise=(1e3, 1e5), mu_max=3.0,
                n_ellipses=10, texture=0.15, chunk=64, input_iters=0, fbp_kind="dd",
-               verbose=False):
+               verbose=False, target="sirt"):
     """(inputs, targets, sinograms, phantoms), each (count, ...) float32 on the GPU.
     ``noise`` = (I0_lo, I0_hi) or None for noiseless data.  The input is the FBP of
     ``fbp_kind`` (``input_iters`` = 0) or the plain SIRT iterate after ``input_iters``
-    iterations, which SIRT then continues to the target."""
+    iterations; the target is the SIRT iterate after ``sirt_iters`` iterations
+    (``target`` = "sirt") or the phantom itself ("truth")."""
+    if target not in ("sirt", "truth"):
+        raise ValueError("target %r (have 'sirt', 'truth')" % target)
     dev = device()
     gen = torch.Generator(device=dev).manual_seed(seed)
     rng = np.random.default_rng(seed)
@@ -130,10 +133,11 @@ def make_pairs(plan, count, sirt_iters, seed=0, noise=(1e3, 1e5), mu_max=3.0,
         if input_iters > 0:
             xi = sirt(plan, b, input_iters)
             ins.append(xi)
-            outs.append(sirt(plan, b, sirt_iters - input_iters, x0=xi))
+            outs.append(x_true if target == "truth" else
+                        sirt(plan, b, sirt_iters - input_iters, x0=xi))
         else:
             ins.append(fbp_start(plan, b, fbp_kind))
-            outs.append(sirt(plan, b, sirt_iters))
+            outs.append(x_true if target == "truth" else sirt(plan, b, sirt_iters))
         sinos.append(b)
         truths.append(x_true)
         if verbose:
@@ -308,6 +312,8 @@ def main(argv=None):
                          "(default: --sirt-iters)")
     ap.add_argument("--fbp", default="dd", choices=("dd", "gridrec"),
                     help="FBP of the network input and of the FBP start")
+    ap.add_argument("--target", default="sirt", choices=("sirt", "truth"),
+                    help="training target: the SIRT iterate after --sirt-iters, or the phantom")
     ap.add_argument("--save", default="")
     args = ap.parse_args(argv)
     plan = dd_plan(args.npix, args.nang)
@@ -316,7 +322,7 @@ def main(argv=None):
     xin, xout, _, _ = make_pairs(plan, args.pairs, args.sirt_iters, seed=1, noise=noise,
                                  n_ellipses=args.ellipses, texture=args.texture,
                                  chunk=args.chunk, input_iters=args.input_iters,
-                                 fbp_kind=args.fbp, verbose=True)
+                                 fbp_kind=args.fbp, verbose=True, target=args.target)
     torch.cuda.synchronize()
     t_data = time.perf_counter() - t0
     net = UNet(seed=0)
@@ -331,7 +337,8 @@ def main(argv=None):
                                  chunk=args.chunk)
     res = compare_starts(plan, net, b_test, args.plain_iters or args.sirt_iters,
                          args.max_iters, input_iters=args.input_iters, fbp_kind=args.fbp)
-    res.update({"pairs": args.pairs, "train_steps": args.steps, "seconds_data": t_data,
+    res.update({"pairs": args.pairs, "train_steps": args.steps, "target": args.target,
+                "seconds_data": t_data,
                 "seconds_train": t_train, "loss_first": float(losses[:10].mean()),
                 "loss_last": float(losses[-10:].mean()), "parameters": net.num_parameters()})
     if args.save:
