@@ -447,12 +447,13 @@ __device__ __forceinline__ float first_pixel(float Dlo, float K, int nh, int npi
 // first_pixel with one correction instead of two, for views whose keys all lie below 2^20
 // in magnitude (every view with at most 32 candidates per bin and line, the staged and
 // the grouped paths: bins narrower than 30.75 pixels and npix <= 32766 bound |D| and |K|
-// by 16384 * 30.75 < 2^19; the chunked path for wider views keeps first_pixel).  There every rounding of a key or of the
-// estimate is below 1/16, so with q the exact index (P_q <= Dlo < P_q+1),
-// floor(Dlo + K + 1/2) is q or q + 1 and one test against P_fe settles it.
-__device__ __forceinline__ float first_pixel_near(float Dlo, float K, int nh, int npix)
+// by 16384 * 30.75 < 2^19; the chunked path for wider views keeps first_pixel).  There every
+// rounding of a key or of the estimate is below 1/32, so with q the exact index
+// (P_q <= Dlo < P_q+1) and Dlh = fl(Dlo + 1/2), rounded once per bin, floor(fl(Dlh + K)) is
+// q or q + 1 and one test against P_fe settles it.
+__device__ __forceinline__ float first_pixel_near(float Dlo, float Dlh, float K, int nh, int npix)
 {
-    float fe = floorf(__fadd_rn(__fadd_rn(Dlo, K), 0.5f));
+    float fe = floorf(__fadd_rn(Dlh, K));
     fe = (Dlo < __fsub_rn(fe, K)) ? __fsub_rn(fe, 1.0f) : fe;
     return __builtin_amdgcn_fmed3f(fe, (float)(-nh), (float)(npix - 1 - nh));
 }
@@ -472,6 +473,7 @@ __device__ __forceinline__ float fwd_grouped(const __amdgpu_buffer_rsrc_t rs, co
                                              bool last_bin, float acc)
 {
     const float xNm1 = (float)(npix - 1 - nh);
+    const float Dlh = __fadd_rn(Dlo, 0.5f);
     float u = line_coord(i_lo, nh);
     int lbase = i_lo * npix;
     int i = i_lo;
@@ -481,7 +483,7 @@ __device__ __forceinline__ float fwd_grouped(const __amdgpu_buffer_rsrc_t rs, co
         for (int l = 0; l < GL; ++l) {
             K[l] = key_K<ARITH>(g, u);
             u = __fadd_rn(u, 1.0f);
-            xq[l] = first_pixel_near(Dlo, K[l], nh, npix);
+            xq[l] = first_pixel_near(Dlo, Dlh, K[l], nh, npix);
             const int q = (int)xq[l] + nh;
             if (ONES || NC <= kFwdMainMaxCand) {
 #pragma unroll
@@ -503,7 +505,7 @@ __device__ __forceinline__ float fwd_grouped(const __amdgpu_buffer_rsrc_t rs, co
     }
     for (; i <= i_hi; ++i, u = __fadd_rn(u, 1.0f), lbase += npix) {
         const float K = key_K<ARITH>(g, u);
-        const float x = first_pixel_near(Dlo, K, nh, npix);
+        const float x = first_pixel_near(Dlo, Dlh, K, nh, npix);
         const int q = (int)x + nh;
         float Dhc = Dhi;
         if (last_bin || Dhi > __fsub_rn(xNm1, K)) Dhc = fminf(Dhi, ls[i]);
@@ -560,6 +562,7 @@ __device__ __forceinline__ float fwd_staged(const DdView& g, const float* __rest
     const int dk = min(d, npix - 1);
     const float Dlo = key_D<ARITH>(g, dk - nh);
     const float Dhi = key_D<ARITH>(g, dk + 1 - nh);
+    const float Dlh = __fadd_rn(Dlo, 0.5f);
     const int d0 = bx * B, d1 = min(d0 + B - 1, npix - 1);
     const float D0lo = key_D<ARITH>(g, d0 - nh);
     const float D1hi = key_D<ARITH>(g, d1 + 1 - nh);
@@ -620,7 +623,7 @@ __device__ __forceinline__ float fwd_staged(const DdView& g, const float* __rest
         for (int l = 0; l < FL; ++l) {
             if (l < ncl) {
                 const float K = key_K<ARITH>(g, u);
-                const float x = first_pixel_near(Dlo, K, nh, npix);
+                const float x = first_pixel_near(Dlo, Dlh, K, nh, npix);
                 // byte offset of the first candidate in the line's window
                 const int ob = min(max(((int)x << 2) + sQb[l], 0), 4 * (W - NC));
                 const float* wp = (const float*)((const char*)&sImg[l * W] + ob);

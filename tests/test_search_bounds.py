@@ -1,7 +1,8 @@
 """The one-test searches of the distance-driven kernels, checked by brute force.
 
 The forward's staged and gather paths find the first pixel of a bin on a line as
-floor(Dlo + K + 1/2), corrected once downwards if Dlo < P_fe; the back walk finds the bin
+floor(fl(Dlo + 1/2) + K) (Dlo + 1/2 rounded once per bin, then the sum rounded per line),
+corrected once downwards if Dlo < P_fe; the back walk finds the bin
 of a pixel's left end lo as floor(lo * est + 1/2), corrected once downwards if lo < D_fd.
 Both rest on a rounding bound: the estimate is the exact index or the one after it, for
 every npix <= 32766 (the forward only for bins narrower than 30.75 pixels, the views it
@@ -44,7 +45,8 @@ def test_one_test_searches_hit_the_exact_index(npix, rinv_target):
         if rinv_target < 30.75:   # forward: first pixel of bin j on the line with offset K
             j = rng.integers(-nh - 1, npix - nh + 1, n).astype(f32)
             D = (j * rinv).astype(f32)
-            e = np.floor(((D + K).astype(f32) + f32(0.5)).astype(f32))
+            Dh = (D + f32(0.5)).astype(f32)
+            e = np.floor((Dh + K).astype(f32))
             e = np.where(D < (e - K).astype(f32), e - 1, e)
             q = largest(lambda q: (q.astype(f32) - K).astype(f32) <= D,
                         np.floor(D.astype(np.float64) + K.astype(np.float64)))
