@@ -145,8 +145,8 @@ def warm_start_leg(plan, b, bnorm, target, x_plain, plain_seconds, max_iters, in
     """The warm-started half of the metric: R/C setup, `input_iters` plain SIRT steps, the
     trained U-Net on that iterate (with plain SIRT's fallback: a non-finite or no-better
     output restarts from zero), then plain SIRT continues to the same residual target.
-    Timed end to end (network weights are loaded and the network is run once before the
-    clock starts, as the one-time set-up).  Also the relative L2 distance of its result
+    Timed end to end (network weights are loaded and the network and its fallback check
+    are run once on a zero input before the clock starts, as the one-time set-up).  Also the relative L2 distance of its result
     to the plain leg's result."""
     import torch
     from iterative_tomographic_reconstruction_algorithms_amd import warmstart as W
@@ -155,7 +155,9 @@ def warm_start_leg(plan, b, bnorm, target, x_plain, plain_seconds, max_iters, in
     dev = b.device
     work = torch.empty((v, n), dtype=torch.float32, device=dev)
     stats = torch.zeros(3, dtype=torch.float64, device=dev)
-    net(torch.zeros((1, n, n), device=dev))            # one-time set-up run
+    # one-time set-up: the network and the fallback check run once on a zero input (every
+    # kernel of the warm start loaded before the clock starts, as the plain leg's are)
+    W.warm_start(plan, net, b[None], x_in=torch.zeros((1, n, n), device=dev), pc_beta=0.0)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     R, C = plan.sirt_weights()
@@ -360,7 +362,7 @@ def main():
                     help="with --extra-configs: the configurations to run (comma list)")
     ap.add_argument("--no-warm-start", dest="warm_start", action="store_false",
                     help="skip the warm-started leg (runs when the trained weights exist)")
-    ap.add_argument("--warm-input-iters", type=int, default=30,
+    ap.add_argument("--warm-input-iters", type=int, default=40,
                     help="plain SIRT steps before the network (the network's input)")
     ap.add_argument("--max-residual-iters", type=int, default=600,
                     help="also time plain SIRT from zero to the residual target, the R/C "
