@@ -133,8 +133,10 @@ int itomo_plan_get_shapes(const itomo_plan *plan, int *fwd_variant, int *back_va
  * replay it in every later projection, bitwise equal to the walks.  No counterpart in
  * the reference (whose radon_rd.c walks every time).  mode: ITOMO_RDM_QUERY reports the
  * cached bytes (0 if none), ITOMO_RDM_BUILD builds it now (synchronous on stream; 0
- * bytes when the geometry does not fit), ITOMO_RDM_DROP frees it and keeps the walks.
- * EINVAL for other projectors. */
+ * bytes when the geometry does not fit or the device is out of memory), ITOMO_RDM_DROP
+ * frees it and keeps the walks (EINVAL once a stream capture has recorded a projection
+ * that replays it: the matrix then lives as long as the plan).  EINVAL for other
+ * projectors. */
 #define ITOMO_RDM_QUERY 0
 #define ITOMO_RDM_BUILD 1
 #define ITOMO_RDM_DROP 2

@@ -104,6 +104,7 @@ struct Plan {
     // a chunk at off[chunk] + 64 j + k, cnt[out] = the output's own number of entries; an
     // entry is {input index, length bits}.
     mutable int rd_mat_state;   // 0: not tried, 1: built, -1: declined
+    mutable int rd_mat_pinned;  // 1 once a stream capture has recorded a launch using it
     mutable int2* rdf_ent;
     mutable int64_t* rdf_off;
     mutable int* rdf_cnt;
@@ -119,6 +120,8 @@ constexpr size_t kRdMatrixMaxBytes = (size_t)8 << 30;
 // scratch region active); synchronous.  Returns ITOMO_OK whether built or declined.
 int rd_matrix_ensure(const Plan& p, hipStream_t st);
 void rd_matrix_free(const Plan& p);
+// Marks the matrix as held by a graph when st is capturing.
+void rd_matrix_note_use(const Plan& p, hipStream_t st);
 // Whether a caller-owned scratch region is active on this thread (no allocation then).
 bool scratch_region_active();
 

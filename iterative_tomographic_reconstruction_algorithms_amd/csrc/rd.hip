@@ -562,6 +562,17 @@ void rd_matrix_free(const Plan& p)
     p.rd_mat_bytes = 0;
 }
 
+// hipMalloc that, when the device is out of memory, leaves no error behind: the matrix is
+// then declined and the walks stay in use.
+template <typename T>
+static bool try_alloc(T** ptr, size_t bytes)
+{
+    if (hipMalloc((void**)ptr, bytes) == hipSuccess) return true;
+    (void)hipGetLastError();
+    *ptr = nullptr;
+    return false;
+}
+
 template <int ARITH>
 static int rd_matrix_build(const Plan& p, hipStream_t st)
 {
@@ -569,9 +580,9 @@ static int rd_matrix_build(const Plan& p, hipStream_t st)
     const int64_t n2 = (int64_t)n * n, nf = (int64_t)V * n;
     const int fcpr = (n + 63) / 64;                       // forward chunks per view
     const RdView* g = (const RdView*)p.d_geom;
-    int rc = check_hip(hipMalloc(&p.rdf_cnt, nf * sizeof(int)), "rd matrix: counts");
-    if (!rc) rc = check_hip(hipMalloc(&p.rdb_cnt, n2 * sizeof(int)), "rd matrix: counts");
-    if (rc) return rc;
+    if (!try_alloc(&p.rdf_cnt, nf * sizeof(int)) || !try_alloc(&p.rdb_cnt, n2 * sizeof(int)))
+        return ITOMO_OK;                                  // declined: the walks stay
+    int rc;
     rd_fwd_record_kernel<ARITH, false><<<dim3(fcpr, V), 64, 0, st>>>(g, n, p.rdf_cnt, nullptr,
                                                                        nullptr);
     rd_back_record_kernel<ARITH, false><<<dim3((n + 15) / 16, (n + 15) / 16), 256, 0, st>>>(
@@ -590,12 +601,13 @@ static int rd_matrix_build(const Plan& p, hipStream_t st)
     const int64_t tb = chunk_layout(cb, n2, (n2 + 63) / 64, n2, &ob);
     const size_t bytes = (size_t)(tf + tb) * sizeof(int2);
     if (bytes > kRdMatrixMaxBytes) return ITOMO_OK;      // declined: the walks stay
-    rc = check_hip(hipMalloc(&p.rdf_ent, std::max<int64_t>(tf, 1) * sizeof(int2)), "rd matrix");
-    if (!rc) rc = check_hip(hipMalloc(&p.rdb_ent, std::max<int64_t>(tb, 1) * sizeof(int2)), "rd matrix");
-    if (!rc) rc = check_hip(hipMalloc(&p.rdf_off, of.size() * sizeof(int64_t)), "rd matrix");
-    if (!rc) rc = check_hip(hipMalloc(&p.rdb_off, ob.size() * sizeof(int64_t)), "rd matrix");
-    if (!rc) rc = check_hip(hipMemcpy(p.rdf_off, of.data(), of.size() * sizeof(int64_t),
-                                      hipMemcpyHostToDevice), "rd matrix");
+    if (!try_alloc(&p.rdf_ent, std::max<int64_t>(tf, 1) * sizeof(int2)) ||
+        !try_alloc(&p.rdb_ent, std::max<int64_t>(tb, 1) * sizeof(int2)) ||
+        !try_alloc(&p.rdf_off, of.size() * sizeof(int64_t)) ||
+        !try_alloc(&p.rdb_off, ob.size() * sizeof(int64_t)))
+        return ITOMO_OK;                                  // declined: the walks stay
+    rc = check_hip(hipMemcpy(p.rdf_off, of.data(), of.size() * sizeof(int64_t),
+                             hipMemcpyHostToDevice), "rd matrix");
     if (!rc) rc = check_hip(hipMemcpy(p.rdb_off, ob.data(), ob.size() * sizeof(int64_t),
                                       hipMemcpyHostToDevice), "rd matrix");
     if (rc) return rc;
@@ -609,6 +621,16 @@ static int rd_matrix_build(const Plan& p, hipStream_t st)
     p.rd_mat_bytes = bytes;
     p.rd_mat_state = 1;
     return ITOMO_OK;
+}
+
+// A launch recorded into a stream capture keeps the matrix's addresses in the graph: from
+// then on the matrix stays until the plan is destroyed (itomo_plan_rd_matrix refuses to
+// drop it).
+void rd_matrix_note_use(const Plan& p, hipStream_t st)
+{
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(st, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone)
+        p.rd_mat_pinned = 1;
 }
 
 int rd_matrix_ensure(const Plan& p, hipStream_t st)
@@ -640,6 +662,7 @@ void launch_rd_fwd(const Plan& p, const float* image, float* out, int nslices,
                    int accumulate, const EpiArgs& e, hipStream_t st)
 {
     dim3 block(64), grid((p.npix + 63) / 64, p.nang, nslices);
+    if (p.rd_mat_state == 1) rd_matrix_note_use(p, st);
     const RdMat mat = p.rd_mat_state == 1
                           ? RdMat{p.rdf_ent, p.rdf_off, p.rdf_cnt}
                           : RdMat{nullptr, nullptr, nullptr};
@@ -696,6 +719,7 @@ static int rd_back_launch(const Plan& p, int ones, const float* sino, float* ima
                           hipStream_t st)
 {
     if (p.rd_mat_state == 1) {
+        rd_matrix_note_use(p, st);
         const int64_t n2 = (int64_t)p.npix * p.npix;
         dim3 mg((unsigned)((n2 + 255) / 256), nslices);
 #define ITOMO_RDM(A, O)                                                                  \
@@ -764,6 +788,9 @@ extern "C" int itomo_plan_rd_matrix(itomo_plan* plan, int mode, size_t* bytes, v
         if (plan->rd_mat_state == -1) plan->rd_mat_state = 0;
         rc = itomo::rd_matrix_ensure(*plan, (hipStream_t)stream);
     } else if (mode == ITOMO_RDM_DROP) {
+        if (plan->rd_mat_pinned)
+            return itomo::set_error(ITOMO_EINVAL, "plan_rd_matrix: a captured graph uses the "
+                                                  "matrix; it stays until the plan is destroyed");
         rc = itomo::check_hip(hipStreamSynchronize((hipStream_t)stream), "plan_rd_matrix: sync");
         itomo::rd_matrix_free(*plan);
         plan->rd_mat_state = -1;

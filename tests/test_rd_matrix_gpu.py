@@ -81,3 +81,31 @@ def test_sirt_replaying_the_matrix_equals_the_walks():
         assert (p.rd_matrix() > 0) == use_matrix
         runs.append(x.cpu().numpy())
     np.testing.assert_array_equal(runs[0], runs[1])
+
+
+def test_graph_keeps_the_matrix_it_replays():
+    # a graph captured after the matrix was built replays it; from then on the matrix
+    # cannot be dropped (it lives as long as the plan), and replays equal eager steps
+    from iterative_tomographic_reconstruction_algorithms_amd import _lib as L
+    from iterative_tomographic_reconstruction_algorithms_amd.batch import Graph
+    n, v = 64, 48
+    ang = (angles_deg(v) * np.pi / 180).astype(np.float32)
+    p = Plan("rd", n, ang)
+    assert p.rd_matrix("build") > 0
+    b = gpu(O.rd_forward(ellipses(n, 6, seed=8), ang, O.FAST))[None].contiguous()
+    R, C = p.sirt_weights()
+    x = torch.zeros((1, n, n), device="cuda")
+    w = torch.empty_like(b)
+    g = Graph.sirt(p, x, b, R, C, w, pc_beta=0.0, iters=2)
+    with pytest.raises(L.ItomoError):
+        p.rd_matrix("drop")
+    assert p.rd_matrix() > 0
+    x_ref = torch.zeros((1, n, n), device="cuda")
+    w_ref = torch.empty_like(b)
+    for rep in range(1, 4):
+        g.launch()
+        for _ in range(2):
+            p.sirt_iterate(x_ref, b, R, C, w_ref, pc_beta=0.0)
+        torch.cuda.synchronize()
+        assert torch.equal(x, x_ref), "replay %d" % rep
+    g.close()
