@@ -236,7 +236,7 @@ __global__ void __launch_bounds__(256)
 conv3x3_tiled_kernel(const bf16_t* __restrict__ X, const bf16_t* __restrict__ Wt,
                      const float* __restrict__ bias, const bf16_t* __restrict__ mask,
                      bf16_t* __restrict__ Y, int H, int W, int Cin, int Cout, int relu,
-                     int co_blocks)
+                     int co_blocks, int vec_out)
 {
     constexpr int TH = 4 * WM, HY = TH + 2, HX = 18, NB = 16 * WN;
     constexpr int NXP = HY * HX * 4;       // 16-byte pieces of a staged halo chunk
@@ -315,6 +315,48 @@ conv3x3_tiled_kernel(const bf16_t* __restrict__ X, const bf16_t* __restrict__ Wt
         }
     }
 
+    if (vec_out) {
+        // the tile's outputs (bias and ReLU applied, rounded to BF16) go through LDS, in
+        // the weights' buffer, as [pixel][channel] rows of NB + 8; then 16-byte pieces
+        // (8 channels of one pixel) are masked and written whole
+        constexpr int OS = NB + 8;
+        static_assert(TH * 16 * OS <= NB * 9 * kConvRow, "output staging fits");
+        __syncthreads();   // every wave is done with the chunk's weights
+#pragma unroll
+        for (int j = 0; j < WN; ++j) {
+            const int col = j * 16 + li;
+            const float bv = (bias && co0 + col < Cout) ? bias[co0 + col] : 0.0f;
+#pragma unroll
+            for (int t = 0; t < WM; ++t)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    float v = acc[t][j][r] + bv;
+                    if (relu) v = fmaxf(v, 0.0f);
+                    sW[((wave * WM + t) * 16 + 4 * lg + r) * OS + col] = f2bf(v);
+                }
+        }
+        __syncthreads();
+        constexpr int NP = TH * 16 * (NB / 8);
+        for (int k = tid; k < NP; k += 256) {
+            const int px = k / (NB / 8), cg = k - px * (NB / 8);
+            const int y = y0 + (px >> 4), x = x0 + (px & 15), co = co0 + 8 * cg;
+            if (y >= H || x >= W || co >= Cout) continue;
+            Frag v;
+            v.u = *(const uint4*)&sW[px * OS + 8 * cg];
+            const int64_t o = (img + (int64_t)y * W + x) * Cout + co;
+            if (mask) {
+                Frag m;
+                m.u = *(const uint4*)(mask + o);
+                bf16_t* vh = (bf16_t*)&v.u;
+                const bf16_t* mh = (const bf16_t*)&m.u;
+#pragma unroll
+                for (int e = 0; e < 8; ++e)
+                    if (!(bf2f(mh[e]) > 0.0f)) vh[e] = 0;
+            }
+            *(uint4*)(Y + o) = v.u;
+        }
+        return;
+    }
 #pragma unroll
     for (int j = 0; j < WN; ++j) {
         const int co = co0 + j * 16 + li;
@@ -934,14 +976,17 @@ int itomo_unet_conv3x3(const void* x, int n, int h, int w, int cin, const void* 
         const int co_blocks = (cout + nb - 1) / nb;
         if ((int64_t)n * co_blocks > 65535) return set_error(ITOMO_EINVAL, "unet_conv3x3: too large");
         dim3 g((unsigned)((w + 15) / 16), (unsigned)((h + th - 1) / th), (unsigned)(n * co_blocks));
+        // 16-byte output stores need whole 8-channel pieces, aligned
+        const int vec_out = cout % 8 == 0 && ((uintptr_t)y % 16) == 0 &&
+                            ((uintptr_t)mask % 16) == 0;
         if (wide)
             conv3x3_tiled_kernel<4, 4><<<g, 256, 0, st>>>((const bf16_t*)x, (const bf16_t*)wt, bias,
                                                           (const bf16_t*)mask, (bf16_t*)y, h, w,
-                                                          cin, cout, relu, co_blocks);
+                                                          cin, cout, relu, co_blocks, vec_out);
         else
             conv3x3_tiled_kernel<4, 2><<<g, 256, 0, st>>>((const bf16_t*)x, (const bf16_t*)wt, bias,
                                                           (const bf16_t*)mask, (bf16_t*)y, h, w,
-                                                          cin, cout, relu, co_blocks);
+                                                          cin, cout, relu, co_blocks, vec_out);
         return check_hip(hipGetLastError(), "unet_conv3x3 launch");
     }
     dim3 grid((unsigned)((P + 127) / 128), (unsigned)((cout + 31) / 32));
