@@ -493,7 +493,8 @@ wgrad_kernel(const bf16_t* __restrict__ X, const bf16_t* __restrict__ dY,
 // M = output channels (a block's 16 WM), N = 9 taps x 32 input channels (one chunk),
 // K = pixels in tiles of 8 x 16 (128).  The MFMA wants 8 consecutive K values per lane,
 // i.e. 8 pixels of one channel, while activations are stored channels-fastest, so both
-// operands are transposed while they are staged: dY of the tile as sD[channel][pixel],
+// operands are transposed (8 x 8 blocks in registers) while they are staged: dY of the
+// tile as sD[channel][pixel],
 // and the tile's 10 x 18 input halo three times, shifted by the tap's column offset, as
 // sX[shift][channel][halo row][16 columns], so that every fragment read is one aligned
 // 16-byte LDS read.  Wave (wr, wc) computes the WM 16-channel tiles from channel 16 WM wr,
@@ -512,9 +513,6 @@ wgrad_tiled_kernel(const bf16_t* __restrict__ X, const bf16_t* __restrict__ dY,
                    int Cin, int Cout, int ci_chunks)
 {
     constexpr int MB = 32 * WM;            // output channels per block (2 waves x WM tiles)
-    constexpr int ND = 128 * MB / 8;       // 16-byte pieces of the staged dY tile
-    constexpr int NX = 10 * 18 * 4;        // 16-byte pieces of the staged halo chunk
-    constexpr int PD = (ND + 255) / 256, PXP = (NX + 255) / 256;
     __shared__ __attribute__((aligned(16))) bf16_t sD[MB * kWgDRow];
     __shared__ __attribute__((aligned(16))) bf16_t sX[3 * 32 * kWgXChan];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -545,59 +543,60 @@ wgrad_tiled_kernel(const bf16_t* __restrict__ X, const bf16_t* __restrict__ dY,
         const int rem = (int)(tile - (int64_t)n * tiles_y * tiles_x);
         const int ty0 = (rem / tiles_x) * kWgTileRows, tx0 = (rem % tiles_x) * 16;
         const int64_t img = (int64_t)n * H * W;
-        // global -> registers: dY pieces (pixel k % 128, channels 8 (k / 128) ..) and
-        // halo pieces (halo pixel k / 4, channels 8 (k % 4) ..); zeros outside
-        uint4 vd[PD], vx[PXP];
-#pragma unroll
-        for (int r = 0; r < PD; ++r) {
-            const int k = tid + r * 256;
-            vd[r] = make_uint4(0, 0, 0, 0);
-            if (k < ND) {
-                const int px = k & 127, cg = k >> 7;
-                const int y = ty0 + (px >> 4), x = tx0 + (px & 15);
-                if (y < H && x < W)
-                    vd[r] = *(const uint4*)(dY + (img + (int64_t)y * W + x) * Cout + co0 + 8 * cg);
-            }
-        }
-#pragma unroll
-        for (int r = 0; r < PXP; ++r) {
-            const int k = tid + r * 256;
-            vx[r] = make_uint4(0, 0, 0, 0);
-            if (k < NX) {
-                const int hp = k >> 2, q = k & 3;
-                const int hy = hp / 18, hx = hp - hy * 18;
-                const int y = ty0 - 1 + hy, x = tx0 - 1 + hx;
-                if (y >= 0 && y < H && x >= 0 && x < W)
-                    vx[r] = *(const uint4*)(X + (img + (int64_t)y * W + x) * Cin + c0 + 8 * q);
-            }
-        }
+        // Staging in units of 8 pixels x 8 channels: a thread loads the 8 pixels' 16-byte
+        // channel pieces, transposes them in registers and writes 8 rows of 8 pixels
+        // as 16-byte LDS stores.  dY units: tile row, half row (8 columns), channel group;
+        // halo units: shifted copy, channel group, halo row, half row (the copy shifted by
+        // sh holds halo columns sh .. sh + 15).  Zeros outside the image.
         __syncthreads();   // the previous tile is consumed
+        constexpr int UD = 16 * (MB / 8);     // dY units
+        constexpr int UX = 3 * 4 * 10 * 2;    // halo units
+        for (int u = tid; u < UD + UX; u += 256) {
+            uint4 in[8];
+            bf16_t* dst;
+            int stride;
+            if (u < UD) {
+                const int rh = u & 15, cg = u >> 4;
+                const int row = rh >> 1, h = rh & 1;
+                const int y = ty0 + row;
 #pragma unroll
-        for (int r = 0; r < PD; ++r) {
-            const int k = tid + r * 256;
-            if (k < ND) {
-                const int px = k & 127, cg = k >> 7;
-                const bf16_t* e = (const bf16_t*)&vd[r];
-#pragma unroll
-                for (int j = 0; j < 8; ++j) sD[(8 * cg + j) * kWgDRow + px] = e[j];
-            }
-        }
-#pragma unroll
-        for (int r = 0; r < PXP; ++r) {
-            const int k = tid + r * 256;
-            if (k < NX) {
-                const int hp = k >> 2, q = k & 3;
-                const int hy = hp / 18, hx = hp - hy * 18;
-                const bf16_t* e = (const bf16_t*)&vx[r];
-#pragma unroll
-                for (int sh = 0; sh < 3; ++sh) {
-                    const int col = hx - sh;   // column of the copy shifted by sh
-                    if (col >= 0 && col < 16) {
-#pragma unroll
-                        for (int j = 0; j < 8; ++j)
-                            sX[(sh * 32 + 8 * q + j) * kWgXChan + hy * kWgXRow + col] = e[j];
-                    }
+                for (int i = 0; i < 8; ++i) {
+                    const int x = tx0 + 8 * h + i;
+                    in[i] = (y < H && x < W)
+                                ? *(const uint4*)(dY + (img + (int64_t)y * W + x) * Cout + co0 + 8 * cg)
+                                : make_uint4(0, 0, 0, 0);
                 }
+                dst = &sD[(8 * cg) * kWgDRow + row * 16 + 8 * h];
+                stride = kWgDRow;
+            } else {
+                const int v = u - UD;
+                const int hh = v % 20, rest = v / 20;      // hh = halo row * 2 + half
+                const int q = rest & 3, sh = rest >> 2;
+                const int hy = hh >> 1, h = hh & 1;
+                const int y = ty0 - 1 + hy;
+#pragma unroll
+                for (int i = 0; i < 8; ++i) {
+                    const int x = tx0 - 1 + sh + 8 * h + i;
+                    in[i] = (y >= 0 && y < H && x >= 0 && x < W)
+                                ? *(const uint4*)(X + (img + (int64_t)y * W + x) * Cin + c0 + 8 * q)
+                                : make_uint4(0, 0, 0, 0);
+                }
+                dst = &sX[(sh * 32 + 8 * q) * kWgXChan + hy * kWgXRow + 8 * h];
+                stride = kWgXChan;
+            }
+            // out row c (channel c of the group) = the 8 pixels' channel c
+#pragma unroll
+            for (int c = 0; c < 8; ++c) {
+                unsigned d[4];
+#pragma unroll
+                for (int m = 0; m < 4; ++m) {
+                    const unsigned* a0 = (const unsigned*)&in[2 * m];
+                    const unsigned* a1 = (const unsigned*)&in[2 * m + 1];
+                    const unsigned lo = a0[c >> 1], hi = a1[c >> 1];
+                    d[m] = (c & 1) ? ((lo >> 16) | (hi & 0xffff0000u))
+                                   : ((lo & 0xffffu) | (hi << 16));
+                }
+                *(uint4*)(dst + c * stride) = make_uint4(d[0], d[1], d[2], d[3]);
             }
         }
         __syncthreads();
