@@ -199,13 +199,14 @@ def _events_ms(fn, reps):
     return a.elapsed_time(b) / reps
 
 
-def extra_configs(cpu=True, which=("cfg1", "cfg2", "cfg3", "cfg4")):
+def extra_configs(cpu=True, which=("cfg1", "cfg2", "cfg3", "cfg4", "unet")):
     """The other workloads of the benchmark suite, measured on the GPU (opt-in):
     cfg1 ray-driven SIRT 256^2 x 304 x 100 iterations (the one-time system-matrix build
     included; the walks' per-iteration time beside it; the reference C timed on two
     iterations), distance-driven MLEM at 2048^2 x 1800 on Poisson counts, a 64-slice sample of the 1024-slice 1024^2 x 900 stack and the mixed
     256-slice launch through the persistent batch (eager and hipGraph), and CG / ADMM-TV
-    per-iteration times at 1024^2 x 720 with 1% Gaussian noise."""
+    per-iteration times at 1024^2 x 720 with 1% Gaussian noise; and the warm start's U-Net
+    forward at 512^2 x 64 and a training step ("unet")."""
     import numpy as np
     import torch
     from iterative_tomographic_reconstruction_algorithms_amd import solvers
@@ -343,6 +344,30 @@ def extra_configs(cpu=True, which=("cfg1", "cfg2", "cfg3", "cfg4")):
             "cg_ms_per_iter": round(t_cg, 3), "admm_ms_per_outer_iter": round(t_admm, 3),
             "cg_seconds_300_iters": round(t_cg * 0.3, 2),
             "admm_seconds_300_iters": round(t_admm * 0.3, 2)}
+    if "unet" in which:
+        # the warm start's network at cfg5's inference shape (512^2, batch 64) and a
+        # training step on 8 crops of 256^2; FLOPs of the 3x3 convolutions only
+        from iterative_tomographic_reconstruction_algorithms_amd.unet import CONVS, UNet
+        net = UNet(seed=0)
+
+        def conv_flops(nb, h, w):
+            lvl = {"0": 1, "1": 2, "2": 4, "3": 8}
+            return sum(2.0 * nb * (h // lvl[nm[1]]) * (w // lvl[nm[1]]) * co * 9 * ci
+                       for nm, ci, co in CONVS)
+        img = torch.rand((64, 512, 512), device=dev)
+        t_fwd = _events_ms(lambda: net(img), 3)
+        del img
+        xi = torch.rand((8, 256, 256), device=dev)
+        yi = torch.rand((8, 256, 256), device=dev)
+        t_step = _events_ms(lambda: net.train_step(xi, yi, lr=1e-4), 20)
+        f_fwd = conv_flops(64, 512, 512)
+        out["unet_512x64"] = {
+            "workload": "U-Net forward, 64 images of 512^2 (1.95 M parameters, BF16 on the "
+                        "matrix cores); training step on 8 crops of 256^2",
+            "forward_ms": round(t_fwd, 3), "conv_tflop_per_forward": round(f_fwd / 1e12, 3),
+            "forward_tflops": round(f_fwd / t_fwd / 1e9, 1),
+            "train_step_ms_8x256": round(t_step, 3),
+            "train_step_tflops_8x256": round(3 * conv_flops(8, 256, 256) / t_step / 1e9, 1)}
     return out
 
 
@@ -358,7 +383,7 @@ def main():
                          "(a kernel of known traffic for scaling FETCH_SIZE / WRITE_SIZE)")
     ap.add_argument("--extra-configs", action="store_true",
                     help="also measure the other configurations of the suite (minutes)")
-    ap.add_argument("--extra-only", default="cfg1,cfg2,cfg3,cfg4",
+    ap.add_argument("--extra-only", default="cfg1,cfg2,cfg3,cfg4,unet",
                     help="with --extra-configs: the configurations to run (comma list)")
     ap.add_argument("--no-warm-start", dest="warm_start", action="store_false",
                     help="skip the warm-started leg (runs when the trained weights exist)")

@@ -830,6 +830,163 @@ __global__ void upcat_bwd_kernel(const bf16_t* __restrict__ dY, const bf16_t* __
     }
 }
 
+// The same four layout kernels on 16-byte pieces of 8 channels, for channel counts that
+// are multiples of 8 and 16-byte aligned tensors: one block row per image row of the
+// batch (blockIdx.y = n * rows + y, 32-bit index arithmetic), threads over (column,
+// channel group).  Each element is computed exactly as in the kernels above.
+__device__ __forceinline__ void unpack8(uint4 u, float* f)
+{
+    const unsigned w[4] = {u.x, u.y, u.z, u.w};
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        f[2 * i] = __uint_as_float(w[i] << 16);
+        f[2 * i + 1] = __uint_as_float(w[i] & 0xffff0000u);
+    }
+}
+
+__device__ __forceinline__ uint4 pack8(const float* f)
+{
+    unsigned w[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) w[i] = f2bf(f[2 * i]) | ((unsigned)f2bf(f[2 * i + 1]) << 16);
+    return make_uint4(w[0], w[1], w[2], w[3]);
+}
+
+__global__ void __launch_bounds__(256)
+pool_fwd_vec_kernel(const bf16_t* __restrict__ X, bf16_t* __restrict__ Y,
+                    unsigned char* __restrict__ idx, int H, int W, int C)
+{
+    const int Ho = H / 2, Wo = W / 2, G = C / 8;
+    const int row = blockIdx.y, n = row / Ho, y = row - n * Ho;
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < Wo * G; t += gridDim.x * blockDim.x) {
+        const int x = t / G, g = t - x * G;
+        float best[8], v[8];
+        unsigned char bi[8];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int yy = 2 * y + (q >> 1), xx = 2 * x + (q & 1);
+            unpack8(*(const uint4*)(X + (((int64_t)n * H + yy) * W + xx) * C + 8 * g), v);
+#pragma unroll
+            for (int e = 0; e < 8; ++e)
+                if (q == 0 || v[e] > best[e]) {
+                    best[e] = v[e];
+                    bi[e] = (unsigned char)q;
+                }
+        }
+        const int64_t o = ((int64_t)row * Wo + x) * C + 8 * g;
+        *(uint4*)(Y + o) = pack8(best);
+        *(uint2*)(idx + o) = make_uint2(bi[0] | (bi[1] << 8) | (bi[2] << 16) | ((unsigned)bi[3] << 24),
+                                        bi[4] | (bi[5] << 8) | (bi[6] << 16) | ((unsigned)bi[7] << 24));
+    }
+}
+
+__global__ void __launch_bounds__(256)
+pool_bwd_vec_kernel(const bf16_t* __restrict__ dY, const unsigned char* __restrict__ idx,
+                    const bf16_t* __restrict__ skip, const bf16_t* __restrict__ Xm,
+                    bf16_t* __restrict__ dX, int H, int W, int C)
+{
+    const int Ho = H / 2, Wo = W / 2, G = C / 8;
+    const int row = blockIdx.y, n = row / H, y = row - n * H;
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < W * G; t += gridDim.x * blockDim.x) {
+        const int x = t / G, g = t - x * G;
+        const int64_t k = ((int64_t)row * W + x) * C + 8 * g;
+        const int64_t o = (((int64_t)n * Ho + y / 2) * Wo + x / 2) * C + 8 * g;
+        float gs[8], dy[8], xm[8];
+        if (skip) unpack8(*(const uint4*)(skip + k), gs);
+        else
+#pragma unroll
+            for (int e = 0; e < 8; ++e) gs[e] = 0.0f;
+        unpack8(*(const uint4*)(dY + o), dy);
+        unpack8(*(const uint4*)(Xm + k), xm);
+        const uint2 iv = *(const uint2*)(idx + o);
+        const unsigned char mine = (unsigned char)((y & 1) * 2 + (x & 1));
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const unsigned char ie = (unsigned char)(((e < 4 ? iv.x : iv.y) >> (8 * (e & 3))) & 0xff);
+            float gv = gs[e];
+            if (ie == mine) gv += dy[e];
+            if (!(xm[e] > 0.0f)) gv = 0.0f;
+            gs[e] = gv;
+        }
+        *(uint4*)(dX + k) = pack8(gs);
+    }
+}
+
+__global__ void __launch_bounds__(256)
+upcat_fwd_vec_kernel(const bf16_t* __restrict__ A, const bf16_t* __restrict__ B,
+                     bf16_t* __restrict__ Y, int H, int W, int Ca, int Cb)
+{
+    const int C = Ca + Cb, G = C / 8;
+    const int row = blockIdx.y, n = row / H, y = row - n * H;
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < W * G; t += gridDim.x * blockDim.x) {
+        const int x = t / G, g = t - x * G;
+        const int64_t pix = (int64_t)row * W + x;
+        const int c = 8 * g;
+        const uint4 v = c < Ca ? *(const uint4*)(A + (((int64_t)n * (H / 2) + y / 2) * (W / 2) + x / 2) * Ca + c)
+                               : *(const uint4*)(B + pix * Cb + (c - Ca));
+        *(uint4*)(Y + pix * C + c) = v;
+    }
+}
+
+// blockIdx.y < N * Ho: the low-resolution rows (dA); above: the full-resolution rows (dB).
+__global__ void __launch_bounds__(256)
+upcat_bwd_vec_kernel(const bf16_t* __restrict__ dY, const bf16_t* __restrict__ A,
+                     bf16_t* __restrict__ dA, bf16_t* __restrict__ dB, int N, int H, int W,
+                     int Ca, int Cb)
+{
+    const int C = Ca + Cb, Ho = H / 2, Wo = W / 2;
+    if ((int)blockIdx.y < N * Ho) {
+        const int row = blockIdx.y, n = row / Ho, y = row - n * Ho, G = Ca / 8;
+        for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < Wo * G;
+             t += gridDim.x * blockDim.x) {
+            const int x = t / G, g = t - x * G;
+            float acc[8], v[8], a[8];
+#pragma unroll
+            for (int e = 0; e < 8; ++e) acc[e] = 0.0f;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int yy = 2 * y + (q >> 1), xx = 2 * x + (q & 1);
+                unpack8(*(const uint4*)(dY + (((int64_t)n * H + yy) * W + xx) * C + 8 * g), v);
+#pragma unroll
+                for (int e = 0; e < 8; ++e) acc[e] += v[e];
+            }
+            const int64_t k = ((int64_t)row * Wo + x) * Ca + 8 * g;
+            unpack8(*(const uint4*)(A + k), a);
+#pragma unroll
+            for (int e = 0; e < 8; ++e)
+                if (!(a[e] > 0.0f)) acc[e] = 0.0f;
+            *(uint4*)(dA + k) = pack8(acc);
+        }
+    } else {
+        const int row = blockIdx.y - N * Ho, G = Cb / 8;
+        for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < W * G; t += gridDim.x * blockDim.x) {
+            const int x = t / G, g = t - x * G;
+            const int64_t pix = (int64_t)row * W + x;
+            *(uint4*)(dB + pix * Cb + 8 * g) = *(const uint4*)(dY + pix * C + Ca + 8 * g);
+        }
+    }
+}
+
+// head_fwd_kernel with the 32 features of a pixel read as four 16-byte pieces.
+__global__ void __launch_bounds__(256)
+head_fwd_vec_kernel(const bf16_t* __restrict__ F, const float* __restrict__ hw,
+                    const float* __restrict__ hb, const float* __restrict__ img,
+                    float* __restrict__ out, int64_t P)
+{
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < P;
+         p += (int64_t)gridDim.x * blockDim.x) {
+        float s = hb[0];
+#pragma unroll
+        for (int c8 = 0; c8 < 4; ++c8) {
+            float f[8];
+            unpack8(*(const uint4*)(F + p * 32 + 8 * c8), f);
+#pragma unroll
+            for (int e = 0; e < 8; ++e) s = fmaf(hw[8 * c8 + e], f[e], s);
+        }
+        out[p] = img[p] + s;
+    }
+}
+
 // ---- head, loss, optimizer ----------------------------------------------------------
 
 // out[p] = img[p] + hb + sum_c hw[c] * F[p][c]   (float result)
@@ -1065,6 +1222,14 @@ int itomo_unet_pool_fwd(const void* x, int n, int h, int w, int c, void* y, void
 {
     if (!x || !y || !idx || n < 1 || h < 2 || w < 2 || c < 1 || (h & 1) || (w & 1))
         return set_error(ITOMO_EINVAL, "unet_pool_fwd: bad argument");
+    if (c % 8 == 0 && (int64_t)n * (h / 2) <= 65535 && ((uintptr_t)x % 16) == 0 &&
+        ((uintptr_t)y % 16) == 0 && ((uintptr_t)idx % 8) == 0) {
+        const int t = (w / 2) * (c / 8);
+        pool_fwd_vec_kernel<<<dim3((unsigned)((t + 255) / 256), (unsigned)(n * (h / 2))), 256, 0,
+                              (hipStream_t)stream>>>((const bf16_t*)x, (bf16_t*)y,
+                                                     (unsigned char*)idx, h, w, c);
+        return check_hip(hipGetLastError(), "unet_pool_fwd launch");
+    }
     const int64_t no = (int64_t)n * (h / 2) * (w / 2) * c;
     pool_fwd_kernel<<<grid_for(no), 256, 0, (hipStream_t)stream>>>(
         (const bf16_t*)x, (bf16_t*)y, (unsigned char*)idx, n, h, w, c);
@@ -1076,6 +1241,16 @@ int itomo_unet_pool_bwd(const void* dy, const void* idx, const void* skip, const
 {
     if (!dy || !idx || !xm || !dx || n < 1 || h < 2 || w < 2 || c < 1 || (h & 1) || (w & 1))
         return set_error(ITOMO_EINVAL, "unet_pool_bwd: bad argument");
+    if (c % 8 == 0 && (int64_t)n * h <= 65535 && ((uintptr_t)dy % 16) == 0 &&
+        ((uintptr_t)idx % 8) == 0 && ((uintptr_t)skip % 16) == 0 && ((uintptr_t)xm % 16) == 0 &&
+        ((uintptr_t)dx % 16) == 0) {
+        const int t = w * (c / 8);
+        pool_bwd_vec_kernel<<<dim3((unsigned)((t + 255) / 256), (unsigned)(n * h)), 256, 0,
+                              (hipStream_t)stream>>>(
+            (const bf16_t*)dy, (const unsigned char*)idx, (const bf16_t*)skip,
+            (const bf16_t*)xm, (bf16_t*)dx, h, w, c);
+        return check_hip(hipGetLastError(), "unet_pool_bwd launch");
+    }
     const int64_t ni = (int64_t)n * h * w * c;
     pool_bwd_kernel<<<grid_for(ni), 256, 0, (hipStream_t)stream>>>(
         (const bf16_t*)dy, (const unsigned char*)idx, (const bf16_t*)skip, (const bf16_t*)xm,
@@ -1088,6 +1263,14 @@ int itomo_unet_upcat_fwd(const void* a, const void* b, int n, int h, int w, int 
 {
     if (!a || !b || !y || n < 1 || h < 2 || w < 2 || ca < 1 || cb < 1 || (h & 1) || (w & 1))
         return set_error(ITOMO_EINVAL, "unet_upcat_fwd: bad argument");
+    if (ca % 8 == 0 && cb % 8 == 0 && (int64_t)n * h <= 65535 && ((uintptr_t)a % 16) == 0 &&
+        ((uintptr_t)b % 16) == 0 && ((uintptr_t)y % 16) == 0) {
+        const int t = w * ((ca + cb) / 8);
+        upcat_fwd_vec_kernel<<<dim3((unsigned)((t + 255) / 256), (unsigned)(n * h)), 256, 0,
+                               (hipStream_t)stream>>>((const bf16_t*)a, (const bf16_t*)b,
+                                                      (bf16_t*)y, h, w, ca, cb);
+        return check_hip(hipGetLastError(), "unet_upcat_fwd launch");
+    }
     const int64_t no = (int64_t)n * h * w * (ca + cb);
     upcat_fwd_kernel<<<grid_for(no), 256, 0, (hipStream_t)stream>>>(
         (const bf16_t*)a, (const bf16_t*)b, (bf16_t*)y, n, h, w, ca, cb);
@@ -1100,6 +1283,15 @@ int itomo_unet_upcat_bwd(const void* dy, const void* a, int n, int h, int w, int
     if (!dy || !a || !da || !db || n < 1 || h < 2 || w < 2 || ca < 1 || cb < 1 || (h & 1) ||
         (w & 1))
         return set_error(ITOMO_EINVAL, "unet_upcat_bwd: bad argument");
+    if (ca % 8 == 0 && cb % 8 == 0 && (int64_t)n * (h / 2) + (int64_t)n * h <= 65535 &&
+        ((uintptr_t)dy % 16) == 0 && ((uintptr_t)a % 16) == 0 && ((uintptr_t)da % 16) == 0 &&
+        ((uintptr_t)db % 16) == 0) {
+        const int t = std::max((w / 2) * (ca / 8), w * (cb / 8));
+        upcat_bwd_vec_kernel<<<dim3((unsigned)((t + 255) / 256), (unsigned)(n * (h / 2) + n * h)),
+                               256, 0, (hipStream_t)stream>>>(
+            (const bf16_t*)dy, (const bf16_t*)a, (bf16_t*)da, (bf16_t*)db, n, h, w, ca, cb);
+        return check_hip(hipGetLastError(), "unet_upcat_bwd launch");
+    }
     const int64_t nt = (int64_t)n * (h / 2) * (w / 2) * ca + (int64_t)n * h * w * cb;
     upcat_bwd_kernel<<<grid_for(nt), 256, 0, (hipStream_t)stream>>>(
         (const bf16_t*)dy, (const bf16_t*)a, (bf16_t*)da, (bf16_t*)db, n, h, w, ca, cb);
@@ -1111,6 +1303,11 @@ int itomo_unet_head_fwd(const void* f, const float* hw, const float* hb, const f
 {
     if (!f || !hw || !hb || !img || !out || npixels < 1 || c < 1 || c > 32)
         return set_error(ITOMO_EINVAL, "unet_head_fwd: bad argument");
+    if (c == 32 && ((uintptr_t)f % 16) == 0) {
+        head_fwd_vec_kernel<<<grid_for(npixels), 256, 0, (hipStream_t)stream>>>(
+            (const bf16_t*)f, hw, hb, img, out, npixels);
+        return check_hip(hipGetLastError(), "unet_head_fwd launch");
+    }
     head_fwd_kernel<<<grid_for(npixels), 256, 0, (hipStream_t)stream>>>(
         (const bf16_t*)f, hw, hb, img, out, npixels, c);
     return check_hip(hipGetLastError(), "unet_head_fwd launch");

@@ -132,8 +132,9 @@ def test_flip_gives_input_gradient():
     assert int(d.max()) <= 1 and float((d > 0).double().mean()) < 0.01
 
 
-def test_pool_upcat_head_exact():
-    n, h, w, c = 2, 8, 12, 32
+@pytest.mark.parametrize("c,ca", [(32, 16), (12, 4)])   # 16-byte piece kernels; element kernels
+def test_pool_upcat_head_exact(c, ca):
+    n, h, w = 2, 8, 12
     e = bf16_rand((n, h, w, c), 9)
     y = torch.empty((n, h // 2, w // 2, c), dtype=torch.bfloat16, device="cuda")
     idx = torch.empty((n, h // 2, w // 2, c), dtype=torch.uint8, device="cuda")
@@ -153,25 +154,25 @@ def test_pool_upcat_head_exact():
     ref = torch.where(e.double() > 0, ref, torch.zeros_like(ref))
     assert torch.equal(dx.cpu(), ref.to(torch.float32).to(torch.bfloat16))
     # upsample + concat and its gradient
-    a = bf16_rand((n, h // 2, w // 2, 16), 12)
-    u = torch.empty((n, h, w, 16 + c), dtype=torch.bfloat16, device="cuda")
+    a = bf16_rand((n, h // 2, w // 2, ca), 12)
+    u = torch.empty((n, h, w, ca + c), dtype=torch.bfloat16, device="cuda")
     ad = a.cuda()
-    L.check(L.lib().itomo_unet_upcat_fwd(P(ad), P(ed), n, h, w, 16, c, P(u),
+    L.check(L.lib().itomo_unet_upcat_fwd(P(ad), P(ed), n, h, w, ca, c, P(u),
                                          stream_handle()), "u")
     ref_u = torch.cat([F.interpolate(a.double().permute(0, 3, 1, 2), scale_factor=2),
                        e.double().permute(0, 3, 1, 2)], dim=1).permute(0, 2, 3, 1)
     assert torch.equal(u.cpu().double(), ref_u)
-    du = bf16_rand((n, h, w, 16 + c), 13)
+    du = bf16_rand((n, h, w, ca + c), 13)
     da = torch.empty_like(a, device="cuda")
     db = torch.empty((n, h, w, c), dtype=torch.bfloat16, device="cuda")
     dud = du.cuda()
-    L.check(L.lib().itomo_unet_upcat_bwd(P(dud), P(ad), n, h, w, 16, c, P(da), P(db),
+    L.check(L.lib().itomo_unet_upcat_bwd(P(dud), P(ad), n, h, w, ca, c, P(da), P(db),
                                          stream_handle()), "ub")
-    ref_da = F.avg_pool2d(du.double().permute(0, 3, 1, 2)[:, :16], 2) * 4
+    ref_da = F.avg_pool2d(du.double().permute(0, 3, 1, 2)[:, :ca], 2) * 4
     ref_da = ref_da.permute(0, 2, 3, 1)
     ref_da = torch.where(a.double() > 0, ref_da, torch.zeros_like(ref_da))
     assert torch.equal(da.cpu(), ref_da.to(torch.float32).to(torch.bfloat16))
-    assert torch.equal(db.cpu(), du[..., 16:])
+    assert torch.equal(db.cpu(), du[..., ca:])
     # head
     f = bf16_rand((n, h, w, 32), 14, positive=True)
     hw = torch.randn(32, generator=torch.Generator().manual_seed(15)).float()
