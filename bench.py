@@ -199,14 +199,15 @@ def _events_ms(fn, reps):
     return a.elapsed_time(b) / reps
 
 
-def extra_configs(cpu=True, which=("cfg1", "cfg2", "cfg3", "cfg4", "unet")):
+def extra_configs(cpu=True, which=("cfg1", "cfg2", "cfg3", "cfg4", "pdss", "unet")):
     """The other workloads of the benchmark suite, measured on the GPU (opt-in):
     cfg1 ray-driven SIRT 256^2 x 304 x 100 iterations (the one-time system-matrix build
     included; the walks' per-iteration time beside it; the reference C timed on two
     iterations), distance-driven MLEM at 2048^2 x 1800 on Poisson counts, a 64-slice sample of the 1024-slice 1024^2 x 900 stack and the mixed
     256-slice launch through the persistent batch (eager and hipGraph), and CG / ADMM-TV
-    per-iteration times at 1024^2 x 720 with 1% Gaussian noise; and the warm start's U-Net
-    forward at 512^2 x 64 and a training step ("unet")."""
+    per-iteration times at 1024^2 x 720 with 1% Gaussian noise; the pixel-driven and
+    slant-stacking pairs at 2048^2 x 1800 ("pdss"); and the warm start's U-Net forward at
+    512^2 x 64 and a training step ("unet")."""
     import numpy as np
     import torch
     from iterative_tomographic_reconstruction_algorithms_amd import solvers
@@ -344,6 +345,23 @@ def extra_configs(cpu=True, which=("cfg1", "cfg2", "cfg3", "cfg4", "unet")):
             "cg_ms_per_iter": round(t_cg, 3), "admm_ms_per_outer_iter": round(t_admm, 3),
             "cg_seconds_300_iters": round(t_cg * 0.3, 2),
             "admm_seconds_300_iters": round(t_admm * 0.3, 2)}
+    if "pdss" in which:
+        # the facade's other two operator pairs at the headline geometry, method 1
+        from iterative_tomographic_reconstruction_algorithms_amd.phantom import angles_deg as _ad
+        n5, v5 = 2048, 1800
+        ang5 = (_ad(v5) * np.pi / 180).astype(np.float32)
+        img5 = torch.from_numpy(ellipses(n5, 40, seed=2)).to(dev)
+        res = {}
+        for oper in ("pd", "ss"):
+            p5 = Plan(oper, n5, ang5, method=1)
+            s5 = p5.forward(img5)
+            t_f = _events_ms(lambda: p5.forward(img5, out=s5), 2)
+            t_b = _events_ms(lambda: p5.back(s5), 2)
+            res[oper] = {"forward_ms": round(t_f, 3), "back_ms": round(t_b, 3),
+                         "gupdates_per_s": round(2.0 * n5 * n5 * v5 / ((t_f + t_b) * 1e-3) / 1e9, 1)}
+            del p5, s5
+        out["pd_ss_2048x1800"] = dict(workload="pixel-driven and slant-stacking pairs (method 1), "
+                                               "2048^2 x 1800, forward and back per call", **res)
     if "unet" in which:
         # the warm start's network at cfg5's inference shape (512^2, batch 64) and a
         # training step on 8 crops of 256^2; FLOPs of the 3x3 convolutions only
@@ -383,7 +401,7 @@ def main():
                          "(a kernel of known traffic for scaling FETCH_SIZE / WRITE_SIZE)")
     ap.add_argument("--extra-configs", action="store_true",
                     help="also measure the other configurations of the suite (minutes)")
-    ap.add_argument("--extra-only", default="cfg1,cfg2,cfg3,cfg4,unet",
+    ap.add_argument("--extra-only", default="cfg1,cfg2,cfg3,cfg4,pdss,unet",
                     help="with --extra-configs: the configurations to run (comma list)")
     ap.add_argument("--no-warm-start", dest="warm_start", action="store_false",
                     help="skip the warm-started leg (runs when the trained weights exist)")
