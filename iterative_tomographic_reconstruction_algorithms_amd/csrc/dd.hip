@@ -59,6 +59,14 @@ transpose_kernel(const float* __restrict__ in, float* __restrict__ out, int npix
     transpose_tile(in, out, npix, stride, blockIdx.x, blockIdx.y, blockIdx.z);
 }
 
+// out = the transposes of nslices npix x npix images (npix^2 apart).
+int launch_transpose(const float* in, float* out, int npix, int nslices, hipStream_t st)
+{
+    dim3 tb(32, 8), tg((npix + 31) / 32, (npix + 31) / 32, nslices);
+    transpose_kernel<<<tg, tb, 0, st>>>(in, out, npix, (int64_t)npix * npix);
+    return check_hip(hipGetLastError(), "transpose launch");
+}
+
 // lstar[v * npix + i] = L* of line i in view v (0 for views without a sweep).
 template <int ARITH>
 __global__ void __launch_bounds__(256)

@@ -120,6 +120,8 @@ constexpr size_t kRdMatrixMaxBytes = (size_t)8 << 30;
 // scratch region active); synchronous.  Returns ITOMO_OK whether built or declined.
 int rd_matrix_ensure(const Plan& p, hipStream_t st);
 void rd_matrix_free(const Plan& p);
+// out = the transposes of nslices npix x npix images (npix^2 apart); stream-ordered.
+int launch_transpose(const float* in, float* out, int npix, int nslices, hipStream_t st);
 // Marks the matrix as held by a graph when st is capturing.
 void rd_matrix_note_use(const Plan& p, hipStream_t st);
 // Whether a caller-owned scratch region is active on this thread (no allocation then).

@@ -164,6 +164,15 @@ __device__ float rd_walk(float acc, const float* __restrict__ img_in, int npix,
     return acc;
 }
 
+// Ray views whose neighbouring bins lie in neighbouring image rows (|sin| > |cos|: sub-ray
+// i lies on y c - x s = tau(i), so one bin further moves a step of (-s, c) in (row,
+// column)) record their forward entries against the transposed image, so that the 64
+// lanes of a wave (64 neighbouring bins) gather from neighbouring addresses.
+__device__ __forceinline__ bool rd_fwd_transposed(const RdView& g)
+{
+    return g.kind == RD_RAYS && fabsf(g.s) > fabsf(g.c);
+}
+
 // The cached system matrix of a plan (see rd_matrix_ensure), or all null.
 struct RdMat {
     const int2* ent;
@@ -217,8 +226,8 @@ __device__ __forceinline__ float rd_replay(float acc, const int2* __restrict__ e
 template <int ARITH, int EPI, bool ONES>
 __global__ void __launch_bounds__(64)
 rd_fwd_kernel(const RdView* __restrict__ geom, int npix, const float* __restrict__ image,
-              float* __restrict__ out, int64_t img_stride, int64_t sino_stride,
-              int accumulate, EpiArgs e, RdMat mat)
+              const float* __restrict__ imageT, float* __restrict__ out, int64_t img_stride,
+              int64_t sino_stride, int accumulate, EpiArgs e, RdMat mat)
 {
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
     const bool active = b < npix;
@@ -231,10 +240,13 @@ rd_fwd_kernel(const RdView* __restrict__ geom, int npix, const float* __restrict
     if (active) {
         float acc = (EPI == EPI_STORE && accumulate) ? out[soff + o] : 0.0f;
         if (mat.ent) {
-            // the bin's cached updates, in order (bitwise the walk's or the sum's)
+            // the bin's cached updates, in order (bitwise the walk's or the sum's), from
+            // the image or its transpose as they were recorded
             const int64_t chunk = (int64_t)v * gridDim.x + blockIdx.x;
+            const float* src = (!ONES && rd_fwd_transposed(g))
+                                   ? imageT + (int64_t)z * img_stride : image;
             acc = rd_replay<ARITH, ONES, false>(acc, mat.ent + mat.off[chunk] + threadIdx.x,
-                                                mat.cnt[(int64_t)v * npix + b], image);
+                                                mat.cnt[(int64_t)v * npix + b], src);
         } else if (g.kind == RD_COLSUM) {
             const int i = npix - 1 - b;  // bin N-1-i <- column i, i >= 1
             if (i >= 1)
@@ -364,7 +376,8 @@ rd_back_gather_kernel(const RdView* __restrict__ geom, int nang, int npix,
 // many times over (8 bytes per walk step, once per direction), and replaying it turns
 // both projections into streaming passes with full occupancy instead of latency-bound
 // walks.  The entries are what the walks compute: per bin the steps of its five sub-rays
-// that enter an image cell, in walk order, {pixel, credited length} (a row-sum or
+// that enter an image cell, in walk order, {pixel, credited length} (the pixel's index in
+// the transposed image for views whose bins step along image rows; a row-sum or
 // column-sum view's bin: its pixels {pixel | 1 << 31}, in order); per pixel, in view
 // order, the row-sum and column-sum views' bins {bin | 1 << 31} and every sub-ray step
 // entering it {bin, length}.  Replaying them in that order with the same update
@@ -379,7 +392,8 @@ __device__ __forceinline__ int rd_sum_flag(int index)
 // One sub-ray walk of radon_rd.c:15-71 (the steps of rd_walk) recording the steps that
 // enter an image cell: FILL == false counts them, FILL == true writes them to out[64 j].
 template <int ARITH, bool FILL>
-__device__ int rd_walk_record(int npix, const float p1[2], const float p2[2], int2* out, int j)
+__device__ int rd_walk_record(int npix, const float p1[2], const float p2[2], int2* out, int j,
+                              bool transposed)
 {
     const int nh = (int)(npix * 0.5);
     const float d0 = __fsub_rn(p2[0], p1[0]), d1 = __fsub_rn(p2[1], p1[1]);
@@ -418,7 +432,9 @@ __device__ int rd_walk_record(int npix, const float p1[2], const float p2[2], in
         const float l = __fmul_rn(__fsub_rn(alpha, alpha_old), incr);
         const int i = ii + nh, jc = jj + nh;
         if (i > 0 && i < npix && jc > 0 && jc < npix) {
-            if (FILL) out[(int64_t)j * 64] = make_int2(i * npix + jc, __float_as_int(l));
+            if (FILL)
+                out[(int64_t)j * 64] =
+                    make_int2(transposed ? jc * npix + i : i * npix + jc, __float_as_int(l));
             ++j;
         }
         alpha_old = alpha;
@@ -451,7 +467,7 @@ rd_fwd_record_kernel(const RdView* __restrict__ geom, int npix, int* __restrict_
         for (int k = 1; k <= 5; ++k) {
             float p1[2], p2[2];
             rd_endpoints<ARITH>(6 * b + k, npix, g, p1, p2);
-            n = rd_walk_record<ARITH, FILL>(npix, p1, p2, out, n);
+            n = rd_walk_record<ARITH, FILL>(npix, p1, p2, out, n, rd_fwd_transposed(g));
         }
     }
     if (!FILL) cnt[(int64_t)v * npix + b] = n;
@@ -658,17 +674,28 @@ int rd_matrix_ensure(const Plan& p, hipStream_t st)
 namespace {
 
 template <int ARITH, int EPI, bool ONES>
-void launch_rd_fwd(const Plan& p, const float* image, float* out, int nslices,
-                   int accumulate, const EpiArgs& e, hipStream_t st)
+int launch_rd_fwd(const Plan& p, const float* image, float* out, int nslices,
+                  int accumulate, const EpiArgs& e, hipStream_t st)
 {
     dim3 block(64), grid((p.npix + 63) / 64, p.nang, nslices);
-    if (p.rd_mat_state == 1) rd_matrix_note_use(p, st);
+    const float* imageT = image;
+    if (p.rd_mat_state == 1) {
+        rd_matrix_note_use(p, st);
+        if (!ONES) {   // the transposed copy the recorded entries of some views index
+            Scratch s;
+            int rc = get_scratch(p, nslices, &s, st);
+            if (!rc) rc = launch_transpose(image, s.xT, p.npix, nslices, st);
+            if (rc) return rc;
+            imageT = s.xT;
+        }
+    }
     const RdMat mat = p.rd_mat_state == 1
                           ? RdMat{p.rdf_ent, p.rdf_off, p.rdf_cnt}
                           : RdMat{nullptr, nullptr, nullptr};
     rd_fwd_kernel<ARITH, EPI, ONES><<<grid, block, 0, st>>>(
-        (const RdView*)p.d_geom, p.npix, image, out, (int64_t)p.npix * p.npix,
+        (const RdView*)p.d_geom, p.npix, image, imageT, out, (int64_t)p.npix * p.npix,
         (int64_t)p.nang * p.npix, accumulate, e, mat);
+    return ITOMO_OK;
 }
 
 template <int ARITH>
@@ -677,21 +704,16 @@ int dispatch_rd_fwd(const Plan& p, int epi, int ones, const float* image, float*
 {
     switch (epi) {
     case EPI_STORE:
-        if (ones) launch_rd_fwd<ARITH, EPI_STORE, true>(p, image, out, nslices, accumulate, e, st);
-        else launch_rd_fwd<ARITH, EPI_STORE, false>(p, image, out, nslices, accumulate, e, st);
-        return ITOMO_OK;
+        if (ones) return launch_rd_fwd<ARITH, EPI_STORE, true>(p, image, out, nslices, accumulate, e, st);
+        return launch_rd_fwd<ARITH, EPI_STORE, false>(p, image, out, nslices, accumulate, e, st);
     case EPI_RECIP:
-        launch_rd_fwd<ARITH, EPI_RECIP, true>(p, image, out, nslices, 0, e, st);
-        return ITOMO_OK;
+        return launch_rd_fwd<ARITH, EPI_RECIP, true>(p, image, out, nslices, 0, e, st);
     case EPI_SIRT_FWD:
-        launch_rd_fwd<ARITH, EPI_SIRT_FWD, false>(p, image, out, nslices, 0, e, st);
-        return ITOMO_OK;
+        return launch_rd_fwd<ARITH, EPI_SIRT_FWD, false>(p, image, out, nslices, 0, e, st);
     case EPI_MLEM_FWD:
-        launch_rd_fwd<ARITH, EPI_MLEM_FWD, false>(p, image, out, nslices, 0, e, st);
-        return ITOMO_OK;
+        return launch_rd_fwd<ARITH, EPI_MLEM_FWD, false>(p, image, out, nslices, 0, e, st);
     case EPI_SPS_FWD:
-        launch_rd_fwd<ARITH, EPI_SPS_FWD, false>(p, image, out, nslices, 0, e, st);
-        return ITOMO_OK;
+        return launch_rd_fwd<ARITH, EPI_SPS_FWD, false>(p, image, out, nslices, 0, e, st);
     }
     return set_error(ITOMO_EINVAL, "rd forward: unsupported epilogue %d", epi);
 }
