@@ -207,7 +207,8 @@ def extra_configs(cpu=True, which=("cfg1", "cfg2", "cfg3", "cfg4", "pdss", "unet
     256-slice launch through the persistent batch (eager and hipGraph), and CG / ADMM-TV
     per-iteration times at 1024^2 x 720 with 1% Gaussian noise; the pixel-driven and
     slant-stacking pairs at 2048^2 x 1800 ("pdss"); and the warm start's U-Net forward at
-    512^2 x 64 and a training step ("unet")."""
+    512^2 x 64 and a training step ("unet").  "cfg3full" (not in the default list: about
+    2 minutes) runs the whole 1024-slice stack for 100 iterations."""
     import numpy as np
     import torch
     from iterative_tomographic_reconstruction_algorithms_amd import solvers
@@ -345,6 +346,37 @@ def extra_configs(cpu=True, which=("cfg1", "cfg2", "cfg3", "cfg4", "pdss", "unet
             "cg_ms_per_iter": round(t_cg, 3), "admm_ms_per_outer_iter": round(t_admm, 3),
             "cg_seconds_300_iters": round(t_cg * 0.3, 2),
             "admm_seconds_300_iters": round(t_admm * 0.3, 2)}
+    if "cfg3full" in which:
+        # the whole stack: 1024 slices of 1024^2 x 900 views, 100 SIRT iterations in one
+        # persistent batch.  The projector's cost does not depend on the content, so the 1024
+        # slices are 16 copies of 64 distinct ellipsoid slices (generating 1024 on the host
+        # takes minutes); b = A x_true on the GPU.
+        n3, v3 = 1024, 900
+        p3 = dd_plan(n3, v3)
+        base = torch.from_numpy(ellipsoid_slices(n3, 1024, 60, seed=3000, first=480, count=64))
+        b3 = torch.empty((1024, v3, n3), dtype=torch.float32, device=dev)
+        for k in range(16):
+            b3[64 * k:64 * (k + 1)] = p3.forward(base.to(dev))
+        del base
+        bt = Batch([Member(p3, b3)], pc_beta=0.0)
+        bt.iterate()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for it in range(100):
+            bt.iterate()
+            if (it + 1) % 10 == 0:
+                torch.cuda.synchronize()
+                print("cfg3full: %d iterations, %.1f s" % (it + 1, time.perf_counter() - t0),
+                      file=sys.stderr, flush=True)
+        torch.cuda.synchronize()
+        secs = time.perf_counter() - t0
+        out["cfg3_full_1024x1024x900"] = {
+            "workload": "persistent batched SIRT, all 1024 slices of 1024^2 x 900 views, "
+                        "100 iterations (after one untimed)",
+            "seconds_100_iters": round(secs, 2), "ms_per_iter": round(secs * 10, 2),
+            "gupdates_per_s": round(2.0 * n3 * n3 * v3 * 1024 * 100 / secs / 1e9, 1)}
+        bt.close()
+        del b3
     if "pdss" in which:
         # the facade's other two operator pairs at the headline geometry, method 1
         from iterative_tomographic_reconstruction_algorithms_amd.phantom import angles_deg as _ad
