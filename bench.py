@@ -444,7 +444,7 @@ def main():
                          "setup included, for at most this many iterations (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (--gpus %d "
+        print("bench.py measures one GPU: the library has no multi-GPU path (--gpus %d "
               "requested)" % args.gpus, file=sys.stderr)
         sys.exit(2)
 

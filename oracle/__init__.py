@@ -1,8 +1,8 @@
 """CPU oracle for the radon projector path.
 
 TEST INFRASTRUCTURE ONLY: imported by ``tests/``, ``__graft_entry__.smoke()`` and the
-``cpu_baseline`` leg of ``bench.py`` as the checker.  The product package never imports
-it and never falls back to it.
+``cpu_baseline`` leg of ``bench.py``, always as the reference the GPU results are compared
+with or timed against.  The product package never imports it and never falls back to it.
 
 * :mod:`oracle.radon`   -- ctypes bindings of ``radon_oracle.c`` (float32 IEEE, float32
   gcc-fast-math form, float64) plus the Cython-wrapper semantics around them.
