@@ -53,6 +53,25 @@ __device__ __forceinline__ bf16x8 load8(const bf16_t* p)
     return f.v;
 }
 
+// 8 BF16 values of one 16-byte piece as floats, and back (round to nearest even)
+__device__ __forceinline__ void unpack8(uint4 u, float* f)
+{
+    const unsigned w[4] = {u.x, u.y, u.z, u.w};
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        f[2 * i] = __uint_as_float(w[i] << 16);
+        f[2 * i + 1] = __uint_as_float(w[i] & 0xffff0000u);
+    }
+}
+
+__device__ __forceinline__ uint4 pack8(const float* f)
+{
+    unsigned w[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) w[i] = f2bf(f[2 * i]) | ((unsigned)f2bf(f[2 * i + 1]) << 16);
+    return make_uint4(w[0], w[1], w[2], w[3]);
+}
+
 __device__ __forceinline__ f32x4 mfma(bf16x8 a, bf16x8 b, f32x4 c)
 {
     return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
@@ -538,68 +557,90 @@ wgrad_tiled_kernel(const bf16_t* __restrict__ X, const bf16_t* __restrict__ dY,
 #pragma unroll
     for (int i = 0; i < WM; ++i) accb[i] = f32x4{0.f, 0.f, 0.f, 0.f};
 
-    for (int64_t tile = blockIdx.z; tile < ntiles; tile += gridDim.z) {
-        const int n = (int)(tile / ((int64_t)tiles_y * tiles_x));
-        const int rem = (int)(tile - (int64_t)n * tiles_y * tiles_x);
-        const int ty0 = (rem / tiles_x) * kWgTileRows, tx0 = (rem % tiles_x) * 16;
-        const int64_t img = (int64_t)n * H * W;
-        // Staging in units of 8 pixels x 8 channels: a thread loads the 8 pixels' 16-byte
-        // channel pieces, transposes them in registers and writes 8 rows of 8 pixels
-        // as 16-byte LDS stores.  dY units: tile row, half row (8 columns), channel group;
-        // halo units: shifted copy, channel group, halo row, half row (the copy shifted by
-        // sh holds halo columns sh .. sh + 15).  Zeros outside the image.
-        __syncthreads();   // the previous tile is consumed
-        constexpr int UD = 16 * (MB / 8);     // dY units
-        constexpr int UX = 3 * 4 * 10 * 2;    // halo units
-        for (int u = tid; u < UD + UX; u += 256) {
-            uint4 in[8];
-            bf16_t* dst;
-            int stride;
-            if (u < UD) {
-                const int rh = u & 15, cg = u >> 4;
-                const int row = rh >> 1, h = rh & 1;
-                const int y = ty0 + row;
+    // Staging in units of 8 channels: a dY unit is 8 pixels of one tile row (tile row,
+    // half row, channel group), a halo unit 10 pixels of one halo row (channel group, halo
+    // row, half row: halo columns 8 h .. 8 h + 9), which give the half rows of all three
+    // shifted copies (the copy shifted by sh holds halo columns sh .. sh + 15).  A thread
+    // has at most one unit (16 MB / 8 + 80 <= 256); it loads the 16-byte channel pieces of
+    // its pixels into registers, transposes them and writes 16-byte LDS rows of 8 pixels of
+    // one channel.  The next tile's loads are issued before the current tile's MFMAs, so
+    // their latency hides behind them.  Zeros outside the image.
+    constexpr int UD = 16 * (MB / 8);     // dY units
+    constexpr int UX = 4 * 10 * 2;        // halo units
+    static_assert(UD + UX <= 256, "one staging unit per thread");
+    const int u = tid;
+    const bool is_d = u < UD, is_x = !is_d && u < UD + UX;
+    // dY unit: tile row ur, half uh, channel group ug; halo unit: halo row ur, half uh,
+    // channel group ug
+    const int ur = is_d ? ((u & 15) >> 1) : (((u - UD) >> 1) % 10);
+    const int uh = is_d ? (u & 1) : ((u - UD) & 1);
+    const int ug = is_d ? (u >> 4) : (((u - UD) >> 1) / 10);
+    // Loads are unconditional, from addresses clamped into the image, and the pieces of
+    // pixels outside it are zeroed when the registers are consumed (bit i of `valid`):
+    // a zero written into a register while a load to it is in flight would make the wave
+    // wait for every load it has issued.
+    const bf16_t* src = is_d ? dY : X;
+    const int cst = is_d ? Cout : Cin;
+    const int cof = (is_d ? co0 : c0) + 8 * ug;
+    const int oy = is_d ? ur : ur - 1, ox = is_d ? 8 * uh : 8 * uh - 1;
+    uint4 in[10];
+    unsigned valid = 0;
+    auto load_unit = [&](int64_t tl) {
+        const int n = (int)(tl / ((int64_t)tiles_y * tiles_x));
+        const int rem = (int)(tl - (int64_t)n * tiles_y * tiles_x);
+        const int y = (rem / tiles_x) * kWgTileRows + oy, x0 = (rem % tiles_x) * 16 + ox;
+        const bool rowok = y >= 0 && y < H;
+        const int64_t row = (int64_t)n * H + min(max(y, 0), H - 1);
+        valid = is_d ? 0x300u : 0u;   // a dY unit has 8 pixels
+        if (is_d || is_x) {
 #pragma unroll
-                for (int i = 0; i < 8; ++i) {
-                    const int x = tx0 + 8 * h + i;
-                    in[i] = (y < H && x < W)
-                                ? *(const uint4*)(dY + (img + (int64_t)y * W + x) * Cout + co0 + 8 * cg)
-                                : make_uint4(0, 0, 0, 0);
+            for (int i = 0; i < 10; ++i) {
+                if (i < 8 || is_x) {
+                    const int x = x0 + i;
+                    in[i] = *(const uint4*)(src + (row * W + min(max(x, 0), W - 1)) * cst + cof);
+                    valid |= (rowok && x >= 0 && x < W) ? (1u << i) : 0u;
                 }
-                dst = &sD[(8 * cg) * kWgDRow + row * 16 + 8 * h];
-                stride = kWgDRow;
-            } else {
-                const int v = u - UD;
-                const int hh = v % 20, rest = v / 20;      // hh = halo row * 2 + half
-                const int q = rest & 3, sh = rest >> 2;
-                const int hy = hh >> 1, h = hh & 1;
-                const int y = ty0 - 1 + hy;
-#pragma unroll
-                for (int i = 0; i < 8; ++i) {
-                    const int x = tx0 - 1 + sh + 8 * h + i;
-                    in[i] = (y >= 0 && y < H && x >= 0 && x < W)
-                                ? *(const uint4*)(X + (img + (int64_t)y * W + x) * Cin + c0 + 8 * q)
-                                : make_uint4(0, 0, 0, 0);
-                }
-                dst = &sX[(sh * 32 + 8 * q) * kWgXChan + hy * kWgXRow + 8 * h];
-                stride = kWgXChan;
             }
-            // out row c (channel c of the group) = the 8 pixels' channel c
+        }
+    };
+    // word m of channel c's row from pixels a (low half) and b (high half)
+    auto pair = [](const uint4& a, const uint4& b, int c) -> unsigned {
+        const unsigned lo = ((const unsigned*)&a)[c >> 1], hi = ((const unsigned*)&b)[c >> 1];
+        return (c & 1) ? ((lo >> 16) | (hi & 0xffff0000u)) : ((lo & 0xffffu) | (hi << 16));
+    };
+    if ((int64_t)blockIdx.z < ntiles) load_unit(blockIdx.z);
+
+    for (int64_t tile = blockIdx.z; tile < ntiles; tile += gridDim.z) {
+        __syncthreads();   // the previous tile is consumed
+        if (valid != 0x3ffu) {
+#pragma unroll
+            for (int i = 0; i < 10; ++i)
+                if (!((valid >> i) & 1u)) in[i] = make_uint4(0, 0, 0, 0);
+        }
+        if (is_d) {
+            bf16_t* dst = &sD[(8 * ug) * kWgDRow + ur * 16 + 8 * uh];
+#pragma unroll
+            for (int c = 0; c < 8; ++c)
+                *(uint4*)(dst + c * kWgDRow) =
+                    make_uint4(pair(in[0], in[1], c), pair(in[2], in[3], c),
+                               pair(in[4], in[5], c), pair(in[6], in[7], c));
+        } else if (is_x) {
+            bf16_t* dst = &sX[(8 * ug) * kWgXChan + ur * kWgXRow + 8 * uh];
 #pragma unroll
             for (int c = 0; c < 8; ++c) {
-                unsigned d[4];
+                // shifts 0 and 2 share the even pairs, shift 1 takes the odd ones
+                unsigned e[5], o[4];
 #pragma unroll
-                for (int m = 0; m < 4; ++m) {
-                    const unsigned* a0 = (const unsigned*)&in[2 * m];
-                    const unsigned* a1 = (const unsigned*)&in[2 * m + 1];
-                    const unsigned lo = a0[c >> 1], hi = a1[c >> 1];
-                    d[m] = (c & 1) ? ((lo >> 16) | (hi & 0xffff0000u))
-                                   : ((lo & 0xffffu) | (hi << 16));
-                }
-                *(uint4*)(dst + c * stride) = make_uint4(d[0], d[1], d[2], d[3]);
+                for (int m = 0; m < 5; ++m) e[m] = pair(in[2 * m], in[2 * m + 1], c);
+#pragma unroll
+                for (int m = 0; m < 4; ++m) o[m] = pair(in[2 * m + 1], in[2 * m + 2], c);
+                *(uint4*)(dst + c * kWgXChan) = make_uint4(e[0], e[1], e[2], e[3]);
+                *(uint4*)(dst + (32 + c) * kWgXChan) = make_uint4(o[0], o[1], o[2], o[3]);
+                *(uint4*)(dst + (64 + c) * kWgXChan) = make_uint4(e[1], e[2], e[3], e[4]);
             }
         }
         __syncthreads();
+        if (tile + gridDim.z < ntiles) load_unit(tile + gridDim.z);
         // K steps of 32 pixels: tile rows 2 s, 2 s + 1; lane group lg holds row
         // 2 s + (lg >> 1), columns 8 (lg & 1) .. + 7
 #pragma unroll
@@ -647,61 +688,71 @@ wgrad_tiled_kernel(const bf16_t* __restrict__ X, const bf16_t* __restrict__ dY,
     }
 }
 
-// The weight and bias gradients of the one-input-channel layer: dW[co][tap] and db[co]
-// over the pixels [blockIdx.x * chunk, +chunk).  Thread (pl, cg) of a block (32 x 8)
-// takes output channels 4 cg .. 4 cg + 3 and the pixels pl, pl + 32, ... of the chunk,
-// walking (image, row, column) incrementally; FP32 sums, added over the 32 pixel lanes in
-// LDS and once atomically.  Cout == 32.
+// The weight and bias gradients of the one-input-channel layer (Cout == 32): dW[co][tap]
+// and db[co].  Work items are 64-pixel segments of one image row; a block takes segments
+// blockIdx.x, + gridDim.x, ...  Per segment the three input rows it needs (66 columns,
+// zeros outside the image) are staged in LDS as floats; thread (pixel px, channel group
+// cq) loads the pixel's 8 dY channels 8 cq .. 8 cq + 7 as one 16-byte piece (four
+// threads read a pixel's 64 bytes, a wave 1 KB in a row) and adds g * x for the 9 taps
+// and g for the bias in FP32.  At the end the lanes of equal channel group are added with
+// shuffles, the four waves in LDS, and one atomic per block and output.
 __global__ void __launch_bounds__(256)
 wgrad_c1_kernel(const bf16_t* __restrict__ X, const bf16_t* __restrict__ dY,
-                float* __restrict__ dW, float* __restrict__ db, int N, int H, int W,
-                int64_t chunk)
+                float* __restrict__ dW, float* __restrict__ db, int N, int H, int W)
 {
-    __shared__ float part[32][32 * 10 + 1];
-    const int pl = threadIdx.x & 31, cg = threadIdx.x >> 5;
-    const int64_t P = (int64_t)N * H * W, hw = (int64_t)H * W;
-    const int64_t pb = (int64_t)blockIdx.x * chunk, pe = min(P, pb + chunk);
-    float a[4][10];
+    __shared__ float sx[3][68];
+    __shared__ float red[4][32 * 10];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int px = tid >> 2, cq = tid & 3;
+    float a[8][10];
 #pragma unroll
-    for (int c = 0; c < 4; ++c)
+    for (int c = 0; c < 8; ++c)
 #pragma unroll
         for (int t = 0; t < 10; ++t) a[c][t] = 0.0f;
-    int64_t p = pb + pl;
-    int64_t n = p / hw;
-    int rem = (int)(p - n * hw);
-    int y = rem / W, x = rem - (rem / W) * W;
-    for (; p < pe; p += 32) {
-        const uint2 gv = *(const uint2*)(dY + p * 32 + 4 * cg);
-        const float g[4] = {__uint_as_float(gv.x << 16), __uint_as_float(gv.x & 0xffff0000u),
-                            __uint_as_float(gv.y << 16), __uint_as_float(gv.y & 0xffff0000u)};
-        const bf16_t* xi = X + n * hw;
+    const int spr = (W + 63) / 64;
+    const int64_t nseg = (int64_t)N * H * spr;
+    for (int64_t sg = blockIdx.x; sg < nseg; sg += gridDim.x) {
+        const int64_t row = sg / spr;               // n * H + y
+        const int x0 = (int)(sg - row * spr) * 64;
+        const int y = (int)(row % H);
+        const int x = x0 + px;
+        // the pixel's gradient first, so that its latency overlaps the staging
+        uint4 gv = make_uint4(0, 0, 0, 0);
+        if (x < W) gv = *(const uint4*)(dY + (row * W + x) * 32 + 8 * cq);
+        __syncthreads();   // the previous segment's rows are consumed
+        if (tid < 3 * 66) {
+            const int r = tid / 66, c = tid - r * 66;
+            const int yy = y + r - 1, xx = x0 + c - 1;
+            sx[r][c] = (yy >= 0 && yy < H && xx >= 0 && xx < W)
+                           ? bf2f(X[(row + r - 1) * W + xx]) : 0.0f;
+        }
+        __syncthreads();
+        float g[8];
+        unpack8(gv, g);
 #pragma unroll
         for (int t = 0; t < 9; ++t) {
-            const int yy = y + t / 3 - 1, xx = x + t % 3 - 1;
-            const float v = (yy >= 0 && yy < H && xx >= 0 && xx < W)
-                                ? bf2f(xi[(int64_t)yy * W + xx]) : 0.0f;
+            const float v = sx[t / 3][px + t % 3];
 #pragma unroll
-            for (int c = 0; c < 4; ++c) a[c][t] = fmaf(g[c], v, a[c][t]);
+            for (int c = 0; c < 8; ++c) a[c][t] = fmaf(g[c], v, a[c][t]);
         }
 #pragma unroll
-        for (int c = 0; c < 4; ++c) a[c][9] += g[c];
-        x += 32;                          // the next pixel of this lane
-        while (x >= W) {
-            x -= W;
-            if (++y == H) {
-                y = 0;
-                ++n;
-            }
-        }
+        for (int c = 0; c < 8; ++c) a[c][9] += g[c];
     }
 #pragma unroll
-    for (int c = 0; c < 4; ++c)
+    for (int off = 4; off < 64; off <<= 1)
 #pragma unroll
-        for (int t = 0; t < 10; ++t) part[pl][(4 * cg + c) * 10 + t] = a[c][t];
+        for (int c = 0; c < 8; ++c)
+#pragma unroll
+            for (int t = 0; t < 10; ++t) a[c][t] += __shfl_xor(a[c][t], off);
+    if (lane < 4) {
+#pragma unroll
+        for (int c = 0; c < 8; ++c)
+#pragma unroll
+            for (int t = 0; t < 10; ++t) red[wave][(8 * lane + c) * 10 + t] = a[c][t];
+    }
     __syncthreads();
-    for (int k = threadIdx.x; k < 32 * 10; k += blockDim.x) {
-        float sum = 0.0f;
-        for (int q = 0; q < 32; ++q) sum += part[q][k];
+    for (int k = tid; k < 32 * 10; k += blockDim.x) {
+        const float sum = (red[0][k] + red[1][k]) + (red[2][k] + red[3][k]);
         const int c = k / 10, t = k - c * 10;
         if (t < 9)
             atomicAdd(&dW[c * 9 + t], sum);
@@ -834,24 +885,6 @@ __global__ void upcat_bwd_kernel(const bf16_t* __restrict__ dY, const bf16_t* __
 // are multiples of 8 and 16-byte aligned tensors: one block row per image row of the
 // batch (blockIdx.y = n * rows + y, 32-bit index arithmetic), threads over (column,
 // channel group).  Each element is computed exactly as in the kernels above.
-__device__ __forceinline__ void unpack8(uint4 u, float* f)
-{
-    const unsigned w[4] = {u.x, u.y, u.z, u.w};
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        f[2 * i] = __uint_as_float(w[i] << 16);
-        f[2 * i + 1] = __uint_as_float(w[i] & 0xffff0000u);
-    }
-}
-
-__device__ __forceinline__ uint4 pack8(const float* f)
-{
-    unsigned w[4];
-#pragma unroll
-    for (int i = 0; i < 4; ++i) w[i] = f2bf(f[2 * i]) | ((unsigned)f2bf(f[2 * i + 1]) << 16);
-    return make_uint4(w[0], w[1], w[2], w[3]);
-}
-
 __global__ void __launch_bounds__(256)
 pool_fwd_vec_kernel(const bf16_t* __restrict__ X, bf16_t* __restrict__ Y,
                     unsigned char* __restrict__ idx, int H, int W, int C)
@@ -1035,6 +1068,56 @@ head_bwd_kernel(const bf16_t* __restrict__ F, const float* __restrict__ hw,
     }
 }
 
+// head_bwd_kernel for C == 32 with 16-byte aligned F and dF: four lanes share a pixel, one
+// 8-channel group each, so a wave moves 1 KB of F and of dF per access.  Every dF element
+// is computed as above; the channel sums are added over the lanes of equal group with
+// shuffles, then over the block's four waves in LDS, one atomic per block and output.
+__global__ void __launch_bounds__(256)
+head_bwd_vec_kernel(const bf16_t* __restrict__ F, const float* __restrict__ hw,
+                    const float* __restrict__ dout, bf16_t* __restrict__ dF,
+                    float* __restrict__ dhw, float* __restrict__ dhb, int64_t P)
+{
+    __shared__ float red[4][33];
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6, c8 = t & 3;
+    float w[8], sw[8];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+        w[e] = hw[8 * c8 + e];
+        sw[e] = 0.0f;
+    }
+    float sb = 0.0f;
+    const int64_t n = 4 * P;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + t; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const float g = dout[i >> 2];
+        if (c8 == 0) sb += g;
+        float f[8], o[8];
+        unpack8(*(const uint4*)(F + 8 * i), f);
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            sw[e] = fmaf(g, f[e], sw[e]);
+            o[e] = f[e] > 0.0f ? g * w[e] : 0.0f;
+        }
+        *(uint4*)(dF + 8 * i) = pack8(o);
+    }
+#pragma unroll
+    for (int off = 4; off < 64; off <<= 1) {
+#pragma unroll
+        for (int e = 0; e < 8; ++e) sw[e] += __shfl_xor(sw[e], off);
+        sb += __shfl_xor(sb, off);
+    }
+    if (lane < 4) {
+#pragma unroll
+        for (int e = 0; e < 8; ++e) red[wave][8 * lane + e] = sw[e];
+        if (lane == 0) red[wave][32] = sb;
+    }
+    __syncthreads();
+    if (t <= 32) {
+        const float s = (red[0][t] + red[1][t]) + (red[2][t] + red[3][t]);
+        atomicAdd(t == 32 ? dhb : &dhw[t], s);
+    }
+}
+
 // d = out - target; grad = 2 d / P; partial sums of d^2 per block (FP64, fixed order).
 __global__ void __launch_bounds__(256)
 mse_kernel(const float* __restrict__ out, const float* __restrict__ target,
@@ -1058,14 +1141,22 @@ mse_kernel(const float* __restrict__ out, const float* __restrict__ target,
     if (threadIdx.x == 0) partials[blockIdx.x] = red[0];
 }
 
-__global__ void sum_partials_kernel(const double* __restrict__ partials, int n, double scale,
-                                    double* __restrict__ out)
+// out = scale * sum of n partials, one block of 256 threads: thread t adds entries t,
+// t + 256, ... in order, then a fixed tree in LDS (the same order on every run).
+__global__ void __launch_bounds__(256)
+sum_partials_kernel(const double* __restrict__ partials, int n, double scale,
+                    double* __restrict__ out)
 {
-    if (threadIdx.x == 0 && blockIdx.x == 0) {
-        double s = 0.0;
-        for (int k = 0; k < n; ++k) s += partials[k];
-        out[0] = s * scale;
+    __shared__ double red[256];
+    double s = 0.0;
+    for (int k = threadIdx.x; k < n; k += 256) s += partials[k];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
     }
+    if (threadIdx.x == 0) out[0] = red[0] * scale;
 }
 
 // Adam with bias correction on FP32 master weights; refreshes the BF16 copy (may be null).
@@ -1173,11 +1264,12 @@ int itomo_unet_wgrad(const void* x, const void* dy, int n, int h, int w, int cin
         return set_error(ITOMO_EINVAL, "unet_wgrad: bad argument");
     const int64_t P = (int64_t)n * h * w;
     hipStream_t st = (hipStream_t)stream;
-    if (cin == 1 && cout == 32 && ((uintptr_t)dy % 8) == 0) {
-        // about 2048 blocks, at least 256 pixels each
-        const int64_t chunk = std::max<int64_t>(256, ((P + 2047) / 2048 + 31) / 32 * 32);
-        wgrad_c1_kernel<<<(unsigned)((P + chunk - 1) / chunk), 256, 0, st>>>(
-            (const bf16_t*)x, (const bf16_t*)dy, dw, db, n, h, w, chunk);
+    if (cin == 1 && cout == 32 && ((uintptr_t)dy % 16) == 0) {
+        // about 2048 blocks, at least 4 row segments each
+        const int64_t nseg = (int64_t)n * h * ((w + 63) / 64);
+        const int64_t g = std::max<int64_t>(1, std::min<int64_t>(2048, (nseg + 3) / 4));
+        wgrad_c1_kernel<<<(unsigned)g, 256, 0, st>>>((const bf16_t*)x, (const bf16_t*)dy, dw, db,
+                                                     n, h, w);
         return check_hip(hipGetLastError(), "unet_wgrad launch");
     }
     if (cin % 32 == 0 && cout % 32 == 0 && ((uintptr_t)x % 16) == 0 && ((uintptr_t)dy % 16) == 0) {
@@ -1318,6 +1410,12 @@ int itomo_unet_head_bwd(const void* f, const float* hw, const float* dout, int64
 {
     if (!f || !hw || !dout || !df || !dhw || !dhb || npixels < 1 || c < 1 || c > 32)
         return set_error(ITOMO_EINVAL, "unet_head_bwd: bad argument");
+    if (c == 32 && ((uintptr_t)f % 16) == 0 && ((uintptr_t)df % 16) == 0) {
+        const unsigned g = grid_for(4 * npixels) < 1024 ? grid_for(4 * npixels) : 1024;
+        head_bwd_vec_kernel<<<g, 256, 0, (hipStream_t)stream>>>(
+            (const bf16_t*)f, hw, dout, (bf16_t*)df, dhw, dhb, npixels);
+        return check_hip(hipGetLastError(), "unet_head_bwd launch");
+    }
     const unsigned g = grid_for(npixels) < 1024 ? grid_for(npixels) : 1024;
     head_bwd_kernel<<<g, 256, 0, (hipStream_t)stream>>>((const bf16_t*)f, hw, dout,
                                                          (bf16_t*)df, dhw, dhb, npixels, c);
@@ -1335,7 +1433,7 @@ int itomo_unet_mse(const float* out, const float* target, int64_t npixels, float
     mse_kernel<<<g, 256, 0, st>>>(out, target, grad, work, npixels);
     int rc = check_hip(hipGetLastError(), "unet_mse launch");
     if (rc) return rc;
-    sum_partials_kernel<<<1, 64, 0, st>>>(work, (int)g, 1.0 / (double)npixels, loss);
+    sum_partials_kernel<<<1, 256, 0, st>>>(work, (int)g, 1.0 / (double)npixels, loss);
     return check_hip(hipGetLastError(), "unet_mse sum launch");
 }
 

@@ -186,6 +186,42 @@ def test_pool_upcat_head_exact(c, ca):
     assert rel(out, ref_o) <= 1e-6
 
 
+@pytest.mark.parametrize("npix,c", [(3 * 40 * 56, 32), (4099, 32), (2 * 9 * 11, 12)])
+def test_head_backward_and_loss(npix, c):
+    # dF is elementwise (bitwise: bf16 of the float product, zero where f <= 0); the
+    # channel and bias sums are FP32 sums in another order: relative 1e-5 against FP64.
+    # 32 channels take the 16-byte kernel, other counts the element kernel.
+    g = torch.Generator().manual_seed(npix + c)
+    f = torch.randn((npix, c), generator=g, dtype=torch.float64).to(torch.bfloat16)
+    hw = torch.randn(c, generator=g).float()
+    dout = torch.randn(npix, generator=g).float()
+    fd, hwd, doutd = f.cuda(), hw.cuda(), dout.cuda()
+    df = torch.empty_like(fd)
+    dhw = torch.full((c,), 0.5, device="cuda")
+    dhb = torch.full((1,), -0.25, device="cuda")
+    L.check(L.lib().itomo_unet_head_bwd(P(fd), P(hwd), P(doutd), npix, c, P(df), P(dhw),
+                                        P(dhb), stream_handle()), "hb")
+    ref_df = torch.where(f.float() > 0, dout[:, None] * hw[None, :], torch.zeros(()))
+    assert torch.equal(df.cpu(), ref_df.to(torch.bfloat16))
+    ref_dhw = 0.5 + (dout.double()[:, None] * f.double()).sum(0)
+    assert rel(dhw, ref_dhw) <= 1e-5
+    assert abs(float(dhb) - (-0.25 + dout.double().sum())) <= 1e-5 * float(dout.double().abs().sum())
+    # the loss: mean squared difference in FP64 and its gradient
+    out = torch.randn(npix, generator=g).float()
+    tgt = torch.randn(npix, generator=g).float()
+    outd, tgtd = out.cuda(), tgt.cuda()
+    grad = torch.empty_like(outd)
+    loss = torch.zeros(1, dtype=torch.float64, device="cuda")
+    work = torch.empty(L.UNET_MSE_WORK, dtype=torch.float64, device="cuda")
+    L.check(L.lib().itomo_unet_mse(P(outd), P(tgtd), npix, P(grad), P(loss), P(work),
+                                   stream_handle()), "mse")
+    d = out - tgt
+    scale = torch.tensor(2.0) / torch.tensor(float(npix))   # float32, as the kernel
+    assert torch.equal(grad.cpu(), scale * d)
+    ref_loss = float((d.double() ** 2).mean())
+    assert abs(float(loss) - ref_loss) <= 1e-12 * ref_loss
+
+
 def _rand_net(seed):
     net = UNet(seed=seed)
     g = torch.Generator().manual_seed(seed + 1)
