@@ -535,16 +535,28 @@ constexpr int kFwdStagedMaxCand = 6;  // views with more candidates take the gat
 constexpr int kFwdStagedWideMaxCand = 32;
 constexpr int kFwdStageMaxLines = 8;
 
+// Pixels of the staged window beyond (NC-1)*B, so that it holds every candidate of a
+// block (see fwd_staged): max(0, NC + 7 - B/4), rounded up to a multiple of 4.
+template <int B, int NC>
+constexpr int fwd_window_extra()
+{
+    return NC + 7 - B / 4 > 0 ? (NC + 7 - B / 4 + 3) / 4 * 4 : 0;
+}
+
 // Forward projection of one sweep view with at most NC candidate pixels per bin and line
-// (bins narrower than NC - 1.25 pixels), for the B bins [bx*B, +B) of a block.
+// (bins narrower than w < NC - 1.25 pixels), for the B bins [bx*B, +B) of a block.
 // Lines are processed in chunks of FL: for each line of a chunk the block first copies
-// the window of W = (NC-1)*B pixels its bins can touch into LDS with 16-byte loads (one
-// contiguous row segment per line; the window starts at the first bin's estimated first
-// pixel minus 2, rounded down to a multiple of 4), then every thread reads its
-// candidates from LDS.  The block's bins and their candidates span at most
-// (B-1)*(NC-1.25) + NC + 1 pixels, so the window holds them (offsets are clamped into it
-// all the same).  Lines run over the union of the block's per-bin line ranges; lines
-// that miss a bin add nothing to it.  Every thread of the block takes part (inactive
+// the window of W pixels its bins can touch into LDS with 16-byte loads (one contiguous
+// row segment per line; the window starts at q0, the first bin's estimated first pixel
+// minus 2, rounded down to a multiple of 4), then every thread reads its candidates from
+// LDS.  A bin's first candidate lies less than B*w + 6 pixels past q0: less than
+// (B-1)*w + 1 + 5 for a bin that starts on the line, and for the last bin up to one bin
+// width more where it starts left of the line's first pixel and its first candidate is
+// clamped up to that pixel.  With W = (NC-1)*B + max(0, NC + 7 - B/4) >= B*w + 6 + NC
+// the window holds every candidate of every bin that meets the line (offsets are
+// clamped into it all the same; a bin that misses the line weights all its candidates
+// by zero).  Lines run over the union of the block's per-bin line ranges; lines that
+// miss a bin add nothing to it.  Every thread of the block takes part (inactive
 // threads, past the last bin, compute a copy of the last bin and store nothing).
 // sImg holds FLOATS floats; sQ0, sQb and sL kFwdStageMaxLines entries.
 template <int ARITH, int B, int NC, int FLOATS>
@@ -553,7 +565,7 @@ __device__ __forceinline__ float fwd_staged(const DdView& g, const float* __rest
                                             int bx, int d, float acc, float* sImg, int* sQ0,
                                             int* sQb, float* sL)
 {
-    constexpr int W = (NC - 1) * B;
+    constexpr int W = (NC - 1) * B + fwd_window_extra<B, NC>();
     constexpr int FL = FLOATS / W < kFwdStageMaxLines ? FLOATS / W : kFwdStageMaxLines;
     constexpr int N4 = FL * (W / 4);          // 16-byte pieces per chunk
     constexpr int PER = (N4 + B - 1) / B;     // pieces per thread

@@ -51,6 +51,34 @@ def test_all_shapes_bit_identical(n, v, nz, arith):
         torch.testing.assert_close(got[5], ref[5], rtol=1e-12, atol=0)
 
 
+def bracket_top_angles():
+    """Views whose bins sit just below the top of each staged candidate bracket (bins
+    w < NC - 1.25 pixels wide for NC = 32, 24, 16, 12, 6, 4), as column sweeps
+    (cos = 1/w) and row sweeps (sin = 1/w), plus the 88.125-degree view of a 1440-view
+    list (w = 30.56), where 64-bin blocks once read shifted candidates for the last bin of
+    a block that starts left of a line."""
+    out = [np.float32(np.pi * 705 / 1440)]
+    for w in (30.74, 30.5, 22.74, 14.74, 10.74, 4.74, 2.74):
+        out.append(np.float32(np.arccos(1.0 / w)))
+        out.append(np.float32(np.pi - np.arcsin(1.0 / w)))
+    return np.array(out, np.float32)
+
+
+@pytest.mark.parametrize("fwd", range(L.FWD_VARIANTS))
+def test_every_forward_shape_at_bracket_tops_matches_oracle(fwd):
+    from oracle import radon as O
+    n = 2048
+    ang = bracket_top_angles()
+    p = Plan("dd", n, ang)
+    p.set_shapes(fwd, p.shapes[1])
+    img = ellipses(n, 12, seed=5)
+    got = p.forward(torch.from_numpy(img).cuda()).cpu().numpy()
+    for v in range(ang.size):
+        want = O.dd_forward(img, ang[v:v + 1], O.FAST)[0]   # one view: no reordering
+        np.testing.assert_array_equal(got[ang.size - 1 - v], want,
+                                      err_msg="variant %d view %d (%.6f rad)" % (fwd, v, ang[v]))
+
+
 def test_bad_shapes_rejected():
     p = dd_plan(32, 8)
     with pytest.raises(L.ItomoError):
