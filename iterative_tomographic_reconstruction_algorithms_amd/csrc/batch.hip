@@ -2,15 +2,12 @@
 // of SIRT iterations.
 //
 // A batch holds members, each one distance-driven plan (npix, angles) with its own stack
-// of slices.  One iteration is: two persistent forward launches (the views with more than
-// 8 candidate pixels per bin and line, then the others, as the per-plan forward splits
-// them), the SIRT update R (b - Ax) as an elementwise pass per member, and one persistent
-// back launch with the SIRT update fused.  A persistent launch is a fixed grid of blocks
-// that pull work items -- a 64-bin block of one view of one slice, or a 64x64 tile of one
-// slice -- from an atomic counter, largest items first, and run the same per-tile device
-// code as the per-plan kernels.  Every output element is still written by one thread in
-// the reference's accumulation order, so results do not depend on the schedule and equal
-// the per-plan path bit for bit.
+// of slices.  One iteration is two persistent launches (forward, back): a fixed grid of
+// 256-thread blocks pulls work items -- a 256-bin block of one view of one slice, or a
+// 64x64 tile of one slice -- from an atomic counter, largest items first, and runs the
+// same per-tile device code as the per-plan kernels.  Every output element is still
+// written by one thread in the reference's accumulation order, so results do not depend
+// on the schedule and equal the per-plan path bit for bit.
 #include <algorithm>
 #include <new>
 #include <vector>
@@ -53,7 +50,7 @@ struct SliceRef {   // for the finalize kernel
 struct itomo_batch {
     int nmembers = 0;
     int total_slices = 0;
-    int nfwd = 0, nwide = 0, nback = 0;   // d_fwd holds the wide-view items first
+    int nfwd = 0, nback = 0;
     int grid = 0;
     std::vector<itomo::BatchMember> h_members;
     std::vector<const itomo_plan*> plans;
@@ -73,7 +70,7 @@ struct itomo_graph {
 
 namespace itomo {
 
-__global__ void reset_counter_kernel(int* c) { c[0] = 0; c[1] = 0; c[2] = 0; }
+__global__ void reset_counter_kernel(int* c) { c[0] = 0; c[1] = 0; }
 
 // Bins per forward work item (one 64-thread block): the per-plan tuner's choice at large
 // sizes; smaller items also balance the persistent grid better than 256-bin ones.
@@ -89,10 +86,6 @@ __device__ __forceinline__ int next_item(int* counter)
     return it;
 }
 
-// The projection A x into work (the SIRT update follows as its own pass); PATH is the
-// per-plan forward's split (kPathWide: views with more than 8 candidates per bin and
-// line, kPathMain: the others), so each launch keeps that launch's registers and LDS.
-template <int PATH>
 __global__ void __launch_bounds__(kBatchFwdBlock)
 batch_fwd_kernel(const BatchMember* __restrict__ mem, const BatchItem* __restrict__ items,
                  int nitems, int* counter)
@@ -102,10 +95,15 @@ batch_fwd_kernel(const BatchMember* __restrict__ mem, const BatchItem* __restric
         if (it >= nitems) break;
         const BatchItem w = items[it];
         const BatchMember m = mem[w.member];
-        const EpiArgs e{};
-        dd_fwd_block<ITOMO_ARITH_FAST, EPI_STORE, false, kBatchFwdBlock, PATH>(
+        EpiArgs e{};
+        e.b = m.b;
+        e.w = m.R;
+        e.partials = m.pf;
+        e.blocks_per_slice = m.bps_f;
+        const int nbx = (m.npix + kBatchFwdBlock - 1) / kBatchFwdBlock;
+        dd_fwd_block<ITOMO_ARITH_FAST, EPI_SIRT_FWD, false, kBatchFwdBlock>(
             m.geom, m.lstar, m.nang, m.npix, m.x, m.xT, m.work, (int64_t)m.npix * m.npix,
-            (int64_t)m.nang * m.npix, 0, e, w.a, w.b, w.z, 0);
+            (int64_t)m.nang * m.npix, 0, e, w.a, w.b, w.z, w.b * nbx + w.a);
     }
 }
 
@@ -114,7 +112,7 @@ batch_back_kernel(const BatchMember* __restrict__ mem, const BatchItem* __restri
                   int nitems, int* counter, float pc_beta)
 {
     for (;;) {
-        const int it = next_item(counter + 2);
+        const int it = next_item(counter + 1);
         if (it >= nitems) break;
         const BatchItem w = items[it];
         const BatchMember m = mem[w.member];
@@ -189,7 +187,7 @@ int itomo_batch_create(itomo_batch** out, int nmembers, const itomo_plan* const*
     itomo_batch* B = new (std::nothrow) itomo_batch();
     if (!B) return set_error(ITOMO_ENOMEM, "batch_create: out of host memory");
     B->nmembers = nmembers;
-    std::vector<BatchItem> fwd, fwd_wide, back;
+    std::vector<BatchItem> fwd, back;
     std::vector<SliceRef> refs;
     std::vector<size_t> cost_f, cost_b;
     size_t scratch_bytes = 0;
@@ -217,9 +215,8 @@ int itomo_batch_create(itomo_batch** out, int nmembers, const itomo_plan* const*
         bm.R = R[m];
         bm.C = C[m];
         bm.work = work[m];
-        // the batch launches its own shapes whatever the plan's choice; the forward
-        // partials come from the elementwise SIRT update pass
-        bm.bps_f = (int)(((size_t)p->nang * p->npix + kEpiChunk - 1) / kEpiChunk);
+        // the batch launches its own shapes whatever the plan's choice
+        bm.bps_f = (int)dd_fwd_slots(p->npix, p->nang, kBatchFwdBlock);
         bm.bps_b = (int)dd_back_blocks(p->npix, kDdBackTile);
         B->h_members.push_back(bm);
         B->plans.push_back(p);
@@ -231,13 +228,9 @@ int itomo_batch_create(itomo_batch** out, int nmembers, const itomo_plan* const*
         scratch_bytes += up(sizeof(double) * 3 * (size_t)bm.bps_b * nslices[m]);
         const int nbf = (p->npix + kBatchFwdBlock - 1) / kBatchFwdBlock;
         const int nbt = (p->npix + kDdBackTile - 1) / kDdBackTile;
-        const DdView* hg = (const DdView*)p->h_geom;
         for (int z = 0; z < nslices[m]; ++z) {
-            for (int v = 0; v < p->nang; ++v) {
-                const bool wide = (hg[v].kind == DD_SWEEP_ROWS || hg[v].kind == DD_SWEEP_COLS) &&
-                                  hg[v].ncand > kFwdMainMaxCand;
-                for (int a = 0; a < nbf; ++a) (wide ? fwd_wide : fwd).push_back({m, a, v, z});
-            }
+            for (int v = 0; v < p->nang; ++v)
+                for (int a = 0; a < nbf; ++a) fwd.push_back({m, a, v, z});
             for (int by = 0; by < nbt; ++by)
                 for (int bx = 0; bx < nbt; ++bx) back.push_back({m, bx, by, z});
             refs.push_back({nullptr, nullptr, bm.bps_f, bm.bps_b,
@@ -246,13 +239,9 @@ int itomo_batch_create(itomo_batch** out, int nmembers, const itomo_plan* const*
         B->total_slices += nslices[m];
     }
     // largest items first: forward items cost ~ npix (lines), back tiles ~ nang (views)
-    auto by_npix = [&](const BatchItem& u, const BatchItem& v) {
+    std::stable_sort(fwd.begin(), fwd.end(), [&](const BatchItem& u, const BatchItem& v) {
         return B->h_members[u.member].npix > B->h_members[v.member].npix;
-    };
-    std::stable_sort(fwd_wide.begin(), fwd_wide.end(), by_npix);
-    std::stable_sort(fwd.begin(), fwd.end(), by_npix);
-    B->nwide = (int)fwd_wide.size();
-    fwd.insert(fwd.begin(), fwd_wide.begin(), fwd_wide.end());
+    });
     std::stable_sort(back.begin(), back.end(), [&](const BatchItem& u, const BatchItem& v) {
         return B->h_members[u.member].nang > B->h_members[v.member].nang;
     });
@@ -275,7 +264,7 @@ int itomo_batch_create(itomo_batch** out, int nmembers, const itomo_plan* const*
     if (!rc) rc = check_hip(hipMalloc(&B->d_fwd, sizeof(BatchItem) * fwd.size()), "batch_create");
     if (!rc) rc = check_hip(hipMalloc(&B->d_back, sizeof(BatchItem) * back.size()), "batch_create");
     if (!rc) rc = check_hip(hipMalloc(&B->d_slices, sizeof(SliceRef) * refs.size()), "batch_create");
-    if (!rc) rc = check_hip(hipMalloc(&B->d_counter, 3 * sizeof(int)), "batch_create");
+    if (!rc) rc = check_hip(hipMalloc(&B->d_counter, 2 * sizeof(int)), "batch_create");
     if (!rc) rc = check_hip(hipMemcpy(B->d_members, B->h_members.data(), sizeof(BatchMember) * nmembers, hipMemcpyHostToDevice), "batch_create");
     if (!rc) rc = check_hip(hipMemcpy(B->d_fwd, fwd.data(), sizeof(BatchItem) * fwd.size(), hipMemcpyHostToDevice), "batch_create");
     if (!rc) rc = check_hip(hipMemcpy(B->d_back, back.data(), sizeof(BatchItem) * back.size(), hipMemcpyHostToDevice), "batch_create");
@@ -315,25 +304,8 @@ int itomo_batch_sirt_iterate(itomo_batch* B, float pc_beta, void* stream)
         transpose_kernel_b<<<tg, tb, 0, st>>>(bm.x, bm.xT, bm.npix, (int64_t)bm.npix * bm.npix);
     }
     reset_counter_kernel<<<1, 1, 0, st>>>(B->d_counter);
-    const int fgrid = B->grid * (256 / kBatchFwdBlock);
-    if (B->nwide > 0)
-        batch_fwd_kernel<kPathWide><<<std::min(fgrid, B->nwide), kBatchFwdBlock, 0, st>>>(
-            B->d_members, B->d_fwd, B->nwide, B->d_counter);
-    if (B->nfwd > B->nwide)
-        batch_fwd_kernel<kPathMain><<<std::min(fgrid, B->nfwd - B->nwide), kBatchFwdBlock, 0,
-                                      st>>>(B->d_members, B->d_fwd + B->nwide,
-                                            B->nfwd - B->nwide, B->d_counter + 1);
-    for (int m = 0; m < B->nmembers; ++m) {
-        const BatchMember& bm = B->h_members[m];
-        EpiArgs e{};
-        e.b = bm.b;
-        e.w = bm.R;
-        e.partials = bm.pf;
-        const int rc = elementwise_fwd_epilogue(EPI_SIRT_FWD, bm.work, e,
-                                                (int64_t)bm.nang * bm.npix, bm.nslices, st,
-                                                nullptr);
-        if (rc) return rc;
-    }
+    batch_fwd_kernel<<<std::min(B->grid * (256 / kBatchFwdBlock), B->nfwd), kBatchFwdBlock, 0,
+                       st>>>(B->d_members, B->d_fwd, B->nfwd, B->d_counter);
     batch_back_kernel<<<std::min(B->grid, B->nback), 256, 0, st>>>(B->d_members, B->d_back,
                                                                      B->nback, B->d_counter,
                                                                      pc_beta);

@@ -45,7 +45,8 @@ def test_batch_of_mixed_geometries_equals_per_plan():
     for m in members:
         x_ref, st_ref = eager_sirt(m.plan, m.b, 4)
         assert torch.equal(m.x, x_ref)
-        # FP64 monitors: the same terms, grouped by each path's launch shapes
+        # FP64 monitors: the batch reduces the residual per wave inside the sweep, the
+        # per-plan path per block of its epilogue pass; same terms, other grouping
         torch.testing.assert_close(m.stats, st_ref, rtol=1e-12, atol=0)
 
 
