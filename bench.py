@@ -15,6 +15,7 @@ projector application).  Prints ONE JSON line.
                   [--residual-target 1e-3] [--max-residual-iters M]
 """
 import argparse
+import ctypes
 import json
 import os
 import sys
@@ -407,6 +408,21 @@ def extra_configs(cpu=True, which=("cfg1", "cfg2", "cfg3", "cfg4", "pdss", "unet
         img = torch.rand((64, 512, 512), device=dev)
         t_fwd = _events_ms(lambda: net(img), 3)
         del img
+        # the weight gradients of every 3x3 layer at the same shape (random BF16 operands)
+        from iterative_tomographic_reconstruction_algorithms_amd import _lib as L
+        from iterative_tomographic_reconstruction_algorithms_amd.core import stream_handle
+        t_wg = 0.0
+        lvl = {"0": 1, "1": 2, "2": 4, "3": 8}
+        for nm, ci, co in CONVS:
+            hh = 512 // lvl[nm[1]]
+            xw = torch.randn((64, hh, hh, ci), device=dev).to(torch.bfloat16)
+            dyw = torch.randn((64, hh, hh, co), device=dev).to(torch.bfloat16)
+            dw = torch.zeros((co, 9, ci), device=dev)
+            dbw = torch.zeros(co, device=dev)
+            pw = [ctypes.c_void_p(t.data_ptr()) for t in (xw, dyw, dw, dbw)]
+            t_wg += _events_ms(lambda: L.check(L.lib().itomo_unet_wgrad(
+                pw[0], pw[1], 64, hh, hh, ci, co, pw[2], pw[3], stream_handle()), "wgrad"), 3)
+            del xw, dyw, dw, dbw
         xi = torch.rand((8, 256, 256), device=dev)
         yi = torch.rand((8, 256, 256), device=dev)
         t_step = _events_ms(lambda: net.train_step(xi, yi, lr=1e-4), 20)
@@ -416,6 +432,7 @@ def extra_configs(cpu=True, which=("cfg1", "cfg2", "cfg3", "cfg4", "pdss", "unet
                         "matrix cores); training step on 8 crops of 256^2",
             "forward_ms": round(t_fwd, 3), "conv_tflop_per_forward": round(f_fwd / 1e12, 3),
             "forward_tflops": round(f_fwd / t_fwd / 1e9, 1),
+            "wgrad_ms": round(t_wg, 3), "wgrad_tflops": round(f_fwd / t_wg / 1e9, 1),
             "train_step_ms_8x256": round(t_step, 3),
             "train_step_tflops_8x256": round(3 * conv_flops(8, 256, 256) / t_step / 1e9, 1)}
     return out
