@@ -94,32 +94,49 @@ void launch_fwd_path(const Plan& p, const float* image, const float* imageT, flo
 }
 
 // Views with more than 8 candidates per bin and line go to a launch of their own (see
-// kPathMain); with unit input, one launch takes every view.
+// kPathMain); with unit input, one launch takes every view.  The wide launch is small
+// (a tenth of the views at 2048^2 x 1800) and its blocks are long, so on its own it
+// leaves most of the GPU idle: it runs on the plan's second stream beside the main launch
+// (both write disjoint sinogram rows), forked from and joined back into `st`.
 template <int ARITH, int EPI, bool ONES, int B>
-void launch_fwd_b(const Plan& p, const float* image, const float* imageT, float* out,
-                  int nslices, int accumulate, const EpiArgs& e, hipStream_t st)
+int launch_fwd_b(const Plan& p, const float* image, const float* imageT, float* out,
+                 int nslices, int accumulate, const EpiArgs& e, hipStream_t st)
 {
     if (ONES || !p.d_vlist || p.n_gather == 0) {
         launch_fwd_path<ARITH, EPI, ONES, B, kPathAll>(p, image, imageT, out, nslices,
                                                        accumulate, e, nullptr, p.nang, st);
-        return;
+        return ITOMO_OK;
     }
+    hipStream_t wide_st = st;
+    if (p.n_staged > 0 && p.aux_stream) {
+        int rc = check_hip(hipEventRecord(p.ev_fork, st), "dd forward: fork");
+        if (!rc) rc = check_hip(hipStreamWaitEvent(p.aux_stream, p.ev_fork, 0), "dd forward: fork");
+        if (rc) return rc;
+        wide_st = p.aux_stream;
+    }
+    launch_fwd_path<ARITH, EPI, ONES, B, kPathWide>(p, image, imageT, out, nslices, accumulate,
+                                                    e, p.d_vlist + p.n_staged, p.n_gather,
+                                                    wide_st);
     if (p.n_staged > 0)
         launch_fwd_path<ARITH, EPI, ONES, B, kPathMain>(p, image, imageT, out, nslices,
                                                         accumulate, e, p.d_vlist, p.n_staged, st);
-    launch_fwd_path<ARITH, EPI, ONES, B, kPathWide>(p, image, imageT, out, nslices, accumulate,
-                                                    e, p.d_vlist + p.n_staged, p.n_gather, st);
+    if (wide_st != st) {
+        int rc = check_hip(hipEventRecord(p.ev_join, wide_st), "dd forward: join");
+        if (!rc) rc = check_hip(hipStreamWaitEvent(st, p.ev_join, 0), "dd forward: join");
+        if (rc) return rc;
+    }
+    return ITOMO_OK;
 }
 
 template <int ARITH, int EPI, bool ONES>
-void launch_fwd(const Plan& p, const float* image, const float* imageT, float* out,
-                int nslices, int accumulate, const EpiArgs& e, hipStream_t st)
+int launch_fwd(const Plan& p, const float* image, const float* imageT, float* out,
+               int nslices, int accumulate, const EpiArgs& e, hipStream_t st)
 {
     static_assert(kFwdBins[0] == 256 && kFwdBins[1] == 128 && kFwdBins[2] == 64, "variants");
     switch (p.fwd_var) {
-    case 1: launch_fwd_b<ARITH, EPI, ONES, 128>(p, image, imageT, out, nslices, accumulate, e, st); break;
-    case 2: launch_fwd_b<ARITH, EPI, ONES, 64>(p, image, imageT, out, nslices, accumulate, e, st); break;
-    default: launch_fwd_b<ARITH, EPI, ONES, 256>(p, image, imageT, out, nslices, accumulate, e, st); break;
+    case 1: return launch_fwd_b<ARITH, EPI, ONES, 128>(p, image, imageT, out, nslices, accumulate, e, st);
+    case 2: return launch_fwd_b<ARITH, EPI, ONES, 64>(p, image, imageT, out, nslices, accumulate, e, st);
+    default: return launch_fwd_b<ARITH, EPI, ONES, 256>(p, image, imageT, out, nslices, accumulate, e, st);
     }
 }
 
@@ -153,15 +170,12 @@ int dispatch_fwd(const Plan& p, int epi, int ones, const float* image, const flo
 {
     switch (epi) {
     case EPI_STORE:
-        if (ones) launch_fwd<ARITH, EPI_STORE, true>(p, image, imageT, out, nslices, accumulate, e, st);
-        else launch_fwd<ARITH, EPI_STORE, false>(p, image, imageT, out, nslices, accumulate, e, st);
-        return ITOMO_OK;
+        if (ones) return launch_fwd<ARITH, EPI_STORE, true>(p, image, imageT, out, nslices, accumulate, e, st);
+        return launch_fwd<ARITH, EPI_STORE, false>(p, image, imageT, out, nslices, accumulate, e, st);
     case EPI_RECIP:
-        launch_fwd<ARITH, EPI_RECIP, true>(p, image, imageT, out, nslices, 0, e, st);
-        return ITOMO_OK;
+        return launch_fwd<ARITH, EPI_RECIP, true>(p, image, imageT, out, nslices, 0, e, st);
     case EPI_MLEM_FWD:
-        launch_fwd<ARITH, EPI_MLEM_FWD, false>(p, image, imageT, out, nslices, 0, e, st);
-        return ITOMO_OK;
+        return launch_fwd<ARITH, EPI_MLEM_FWD, false>(p, image, imageT, out, nslices, 0, e, st);
     }
     return set_error(ITOMO_EINVAL, "dd forward: unsupported epilogue %d", epi);
 }

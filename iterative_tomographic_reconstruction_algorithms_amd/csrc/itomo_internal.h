@@ -91,6 +91,11 @@ struct Plan {
                      // to the image
     int* d_vlist;    // distance-driven: views of the main forward launch, then the wide ones
     int n_staged, n_gather;  // (more than 8 candidate pixels per bin and line)
+    // Distance-driven plans with both kinds of views: the wide views' forward launch runs
+    // on this second stream (default priority), forked from and joined back into the
+    // caller's stream by the two events, beside the main launch.
+    hipStream_t aux_stream;
+    hipEvent_t ev_fork, ev_join;
     int fwd_var;     // distance-driven launch shapes (kFwdBins / kBackTiles index)
     int back_var;
     int method;      // pixel-driven: 0 nearest bin, 1 linear split (radon_pd.c:57-85)

@@ -377,6 +377,16 @@ static int plan_create(itomo_plan** out, int projector, int npix, const float* a
                        "plan_create: hipMemcpy");
     if (st == ITOMO_OK && projector == ITOMO_PROJ_DD) st = dd_build_line_ends(*p);
     if (st == ITOMO_OK && projector == ITOMO_PROJ_DD) st = dd_build_view_lists(*p);
+    if (st == ITOMO_OK && projector == ITOMO_PROJ_DD && p->n_staged > 0 && p->n_gather > 0) {
+        st = check_hip(hipStreamCreateWithFlags(&p->aux_stream, hipStreamNonBlocking),
+                       "plan_create: stream");
+        if (st == ITOMO_OK)
+            st = check_hip(hipEventCreateWithFlags(&p->ev_fork, hipEventDisableTiming),
+                           "plan_create: event");
+        if (st == ITOMO_OK)
+            st = check_hip(hipEventCreateWithFlags(&p->ev_join, hipEventDisableTiming),
+                           "plan_create: event");
+    }
     if (st == ITOMO_OK && projector == ITOMO_PROJ_SS) {
         const size_t n = (size_t)nang * npix * 2;
         std::vector<int> tab;
@@ -397,6 +407,9 @@ static int plan_create(itomo_plan** out, int projector, int npix, const float* a
                            "plan_create: hipMemcpy");
     }
     if (st != ITOMO_OK) {
+        if (p->ev_join) (void)hipEventDestroy(p->ev_join);
+        if (p->ev_fork) (void)hipEventDestroy(p->ev_fork);
+        if (p->aux_stream) (void)hipStreamDestroy(p->aux_stream);
         if (p->d_vlist) (void)hipFree(p->d_vlist);
         if (p->d_range) (void)hipFree(p->d_range);
         if (p->d_lstar) (void)hipFree(p->d_lstar);
@@ -442,6 +455,12 @@ int itomo_plan_destroy(itomo_plan* p)
     }
     if (p->d_vlist) {
         int st2 = check_hip(hipFree(p->d_vlist), "plan_destroy: hipFree");
+        if (!st) st = st2;
+    }
+    if (p->ev_join) (void)hipEventDestroy(p->ev_join);
+    if (p->ev_fork) (void)hipEventDestroy(p->ev_fork);
+    if (p->aux_stream) {
+        int st2 = check_hip(hipStreamDestroy(p->aux_stream), "plan_destroy: stream");
         if (!st) st = st2;
     }
     if (p->scratch) {
