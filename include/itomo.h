@@ -98,6 +98,11 @@ int itomo_radon_ss(float *image, int npix, const float *angles, int nang, int op
  * Distance-driven plans reject (ITOMO_EANGLE) views that sweep rows with sin <= 0 or
  * columns with cos <= 0: for those the reference's sweep indexes bin -1 (undefined
  * behaviour in radon_dd.c:158-188).  Angles in [0, pi) are always accepted.
+ * A distance-driven plan whose views include both narrow ones (at most 8 candidate pixels
+ * per bin and line) and wider ones also creates a HIP stream (default priority) and two
+ * events: its forward projections run the wider views' launch on that stream, forked
+ * from the caller's stream and joined back into it, so callers see one ordered stream
+ * (hipGraph capture records the fork and join).
  * Replaces class_projectors_radon.projectors.__init__ (class_projectors_radon.py:47-55).
  * ------------------------------------------------------------------------------- */
 int itomo_plan_create(itomo_plan **plan, int projector, int npix, const float *angles,
