@@ -114,12 +114,14 @@ int launch_fwd_b(const Plan& p, const float* image, const float* imageT, float* 
         if (rc) return rc;
         wide_st = p.aux_stream;
     }
-    launch_fwd_path<ARITH, EPI, ONES, B, kPathWide>(p, image, imageT, out, nslices, accumulate,
-                                                    e, p.d_vlist + p.n_staged, p.n_gather,
-                                                    wide_st);
+    // the main launch is issued first: the wide launch's long blocks then take the slots
+    // the main launch's blocks leave, rather than holding slots from its start
     if (p.n_staged > 0)
         launch_fwd_path<ARITH, EPI, ONES, B, kPathMain>(p, image, imageT, out, nslices,
                                                         accumulate, e, p.d_vlist, p.n_staged, st);
+    launch_fwd_path<ARITH, EPI, ONES, B, kPathWide>(p, image, imageT, out, nslices, accumulate,
+                                                    e, p.d_vlist + p.n_staged, p.n_gather,
+                                                    wide_st);
     if (wide_st != st) {
         int rc = check_hip(hipEventRecord(p.ev_join, wide_st), "dd forward: join");
         if (!rc) rc = check_hip(hipStreamWaitEvent(st, p.ev_join, 0), "dd forward: join");
