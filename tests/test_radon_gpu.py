@@ -119,6 +119,36 @@ def test_dd_is_deterministic():
     assert torch.equal(b1, b2)
 
 
+def test_dd_forward_on_a_side_stream_equals_the_default_stream():
+    # the wide views' launch runs on the plan's own stream, forked from and joined back
+    # into the caller's: called from another torch stream, forward, back and a SIRT step
+    # equal those on the default stream, also when the next call reuses the output at once
+    n, v = 192, 180          # 1-degree steps: views next to 90 degrees have > 8 candidates
+    ang = np.fft.fftshift((angles_deg(v) * np.pi / 180).astype(np.float32))
+    p = Plan("dd", n, ang)
+    img = gpu(np.stack([ellipses(n, 10, seed=s) for s in (6, 7)]))
+    ref = p.forward(img)
+    R, C = p.sirt_weights()
+    x_ref = torch.zeros_like(img)
+    w_ref = torch.empty_like(ref)
+    p.sirt_iterate(x_ref, ref, R, C, w_ref)
+    torch.cuda.synchronize()
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        got = p.forward(img)
+        again = p.back(got)            # consumes the forward's output on the same stream
+        x = torch.zeros_like(img)
+        w = torch.empty_like(got)
+        p.sirt_iterate(x, got, R, C, w)
+    side.synchronize()
+    assert torch.equal(got, ref)
+    assert torch.equal(again, p.back(ref))
+    assert torch.equal(x, x_ref)
+    np.testing.assert_array_equal(got[0].cpu().numpy(), O.dd_forward(ellipses(n, 10, seed=6),
+                                                                      np.fft.ifftshift(ang)))
+
+
 def test_dropin_modules_numpy_roundtrip_and_errors():
     n, v = 32, 12
     deg = angles_deg(v).astype(np.float32)
