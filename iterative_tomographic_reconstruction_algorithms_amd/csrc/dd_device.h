@@ -596,7 +596,7 @@ __device__ __forceinline__ float fwd_staged(const DdView& g, const float* __rest
                                     (float)(npix - nh + W));
             const int q0 = ((int)est + nh - 2) & ~3;
             sQ0[tid] = q0;
-            sQb[tid] = 4 * (nh - q0);
+            sQb[tid] = 4 * (nh - q0 + tid * W);   // byte offset of pixel 0 in sImg
             sL[tid] = ls[c0 + tid];
         }
         __syncthreads();
@@ -636,9 +636,9 @@ __device__ __forceinline__ float fwd_staged(const DdView& g, const float* __rest
             if (l < ncl) {
                 const float K = key_K<ARITH>(g, u);
                 const float x = first_pixel_near(Dlo, Dlh, K, nh, npix);
-                // byte offset of the first candidate in the line's window
-                const int ob = min(max(((int)x << 2) + sQb[l], 0), 4 * (W - NC));
-                const float* wp = (const float*)((const char*)&sImg[l * W] + ob);
+                // byte offset of the first candidate in sImg, clamped into the line's window
+                const int ob = min(max(((int)x << 2) + sQb[l], 4 * l * W), 4 * (l * W + W - NC));
+                const float* wp = (const float*)((const char*)sImg + ob);
                 // clipping every weight's right end to L* is the exact rule; it changes
                 // only the weights that end at the line's dropped segment
                 const float Dhc = fminf(Dhi, sL[l]);
