@@ -335,6 +335,8 @@ def extra_configs(cpu=True, which=("cfg1", "cfg2", "cfg3", "cfg4", "pdss", "unet
         # cfg4: CG and ADMM-TV at 1024^2 x 720, Gaussian noise of 1% of the largest value
         n4, v4 = 1024, 720
         p4 = dd_plan(n4, v4)
+        from iterative_tomographic_reconstruction_algorithms_amd import autotune
+        shapes4 = autotune.tune(p4, nslices=1, reps=2, use_cache=False)
         b4 = p4.forward(torch.from_numpy(ellipses(n4, 10, seed=4)).to(dev))
         gen = torch.Generator(device=dev).manual_seed(4)
         b4 = (b4 + 0.01 * float(b4.max()) * torch.randn(b4.shape, device=dev, generator=gen)).contiguous()
@@ -344,6 +346,7 @@ def extra_configs(cpu=True, which=("cfg1", "cfg2", "cfg3", "cfg4", "pdss", "unet
         out["cfg4_cg_admm_1024x720"] = {
             "workload": "CG and ADMM-TV (4 inner CG steps per outer step), 1024^2 x 720, "
                         "1% Gaussian noise, FP64 scalars on the GPU",
+            "shapes": {"fwd": shapes4["fwd"], "back": shapes4["back"]},
             "cg_ms_per_iter": round(t_cg, 3), "admm_ms_per_outer_iter": round(t_admm, 3),
             "cg_seconds_300_iters": round(t_cg * 0.3, 2),
             "admm_seconds_300_iters": round(t_admm * 0.3, 2)}
