@@ -86,7 +86,9 @@ __device__ __forceinline__ int next_item(int* counter)
     return it;
 }
 
-__global__ void __launch_bounds__(kBatchFwdBlock)
+// Four waves per SIMD: the kernel holds every forward path and the fused update, and
+// left to itself the compiler takes 129 registers, which leaves room for only three.
+__global__ void __launch_bounds__(kBatchFwdBlock) __attribute__((amdgpu_waves_per_eu(4)))
 batch_fwd_kernel(const BatchMember* __restrict__ mem, const BatchItem* __restrict__ items,
                  int nitems, int* counter)
 {
@@ -100,10 +102,9 @@ batch_fwd_kernel(const BatchMember* __restrict__ mem, const BatchItem* __restric
         e.w = m.R;
         e.partials = m.pf;
         e.blocks_per_slice = m.bps_f;
-        const int nbx = (m.npix + kBatchFwdBlock - 1) / kBatchFwdBlock;
         dd_fwd_block<ITOMO_ARITH_FAST, EPI_SIRT_FWD, false, kBatchFwdBlock>(
             m.geom, m.lstar, m.nang, m.npix, m.x, m.xT, m.work, (int64_t)m.npix * m.npix,
-            (int64_t)m.nang * m.npix, 0, e, w.a, w.b, w.z, w.b * nbx + w.a);
+            (int64_t)m.nang * m.npix, 0, e, w.a, w.b, w.z);
     }
 }
 
@@ -216,7 +217,7 @@ int itomo_batch_create(itomo_batch** out, int nmembers, const itomo_plan* const*
         bm.C = C[m];
         bm.work = work[m];
         // the batch launches its own shapes whatever the plan's choice
-        bm.bps_f = (int)dd_fwd_slots(p->npix, p->nang, kBatchFwdBlock);
+        bm.bps_f = (int)dd_fwd_slots(p->npix, p->nang);
         bm.bps_b = (int)dd_back_blocks(p->npix, kDdBackTile);
         B->h_members.push_back(bm);
         B->plans.push_back(p);

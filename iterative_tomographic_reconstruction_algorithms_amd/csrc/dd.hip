@@ -38,19 +38,19 @@ dd_back_kernel(const DdView* __restrict__ geom, const float* __restrict__ lstar,
 }
 
 // views: the view of each blockIdx.y (null: blockIdx.y itself).  FP64 partial slots are
-// one per 64 bins, numbered by view (blocks_per_view blocks of B bins per view in the
-// table), so launches over disjoint view lists, also with different B, fill one table.
+// one per 64-bin chunk, numbered by view (dd_fwd_slots), so launches over disjoint view
+// lists, also with different B, fill one table.
 template <int ARITH, int EPI, bool ONES, int B, int PATH>
 __global__ void __launch_bounds__(B)
 dd_fwd_kernel(const DdView* __restrict__ geom, const float* __restrict__ lstar, int nang,
               int npix, const float* __restrict__ image, const float* __restrict__ imageT,
               float* __restrict__ out, int64_t img_stride, int64_t sino_stride,
-              int accumulate, EpiArgs e, const int* __restrict__ views, int blocks_per_view)
+              int accumulate, EpiArgs e, const int* __restrict__ views)
 {
     const int view = views ? views[blockIdx.y] : (int)blockIdx.y;
     dd_fwd_block<ARITH, EPI, ONES, B, PATH>(geom, lstar, nang, npix, image, imageT, out,
                                             img_stride, sino_stride, accumulate, e, blockIdx.x,
-                                            view, blockIdx.z, view * blocks_per_view + blockIdx.x);
+                                            view, blockIdx.z);
 }
 
 __global__ void __launch_bounds__(256)
@@ -83,18 +83,15 @@ line_end_kernel(const DdView* __restrict__ geom, int nang, int npix, float* __re
 
 namespace {
 
-// slots_per_view: FP64 partial slots (64 bins each) per view in the slot table, that of
-// the plan's forward shape (fwd_blocks_per_slice / nang).
 template <int ARITH, int EPI, bool ONES, int B, int PATH>
 void launch_fwd_path(const Plan& p, const float* image, const float* imageT, float* out,
                      int nslices, int accumulate, const EpiArgs& e, const int* views,
-                     int nviews, int slots_per_view, hipStream_t st)
+                     int nviews, hipStream_t st)
 {
     dim3 block(B), grid((p.npix + B - 1) / B, nviews, nslices);
     dd_fwd_kernel<ARITH, EPI, ONES, B, PATH><<<grid, block, 0, st>>>(
         (const DdView*)p.d_geom, p.d_lstar, p.nang, p.npix, image, imageT, out,
-        (int64_t)p.npix * p.npix, (int64_t)p.nang * p.npix, accumulate, e, views,
-        slots_per_view / (B / 64));
+        (int64_t)p.npix * p.npix, (int64_t)p.nang * p.npix, accumulate, e, views);
 }
 
 // Views with more than 8 candidate pixels per bin and line go to a launch of their own
@@ -109,11 +106,9 @@ template <int ARITH, int EPI, bool ONES, int B>
 int launch_fwd_b(const Plan& p, const float* image, const float* imageT, float* out,
                  int nslices, int accumulate, const EpiArgs& e, hipStream_t st)
 {
-    const int slots_per_view = (p.npix + B - 1) / B * (B / 64);
     if (ONES || !p.d_vlist || p.n_gather == 0) {
         launch_fwd_path<ARITH, EPI, ONES, B, kPathAll>(p, image, imageT, out, nslices,
-                                                       accumulate, e, nullptr, p.nang,
-                                                       slots_per_view, st);
+                                                       accumulate, e, nullptr, p.nang, st);
         return ITOMO_OK;
     }
     hipStream_t wide_st = st;
@@ -127,11 +122,10 @@ int launch_fwd_b(const Plan& p, const float* image, const float* imageT, float* 
     // the main launch's blocks leave, rather than holding slots from its start
     if (p.n_staged > 0)
         launch_fwd_path<ARITH, EPI, ONES, B, kPathMain>(p, image, imageT, out, nslices,
-                                                        accumulate, e, p.d_vlist, p.n_staged,
-                                                        slots_per_view, st);
+                                                        accumulate, e, p.d_vlist, p.n_staged, st);
     launch_fwd_path<ARITH, EPI, ONES, 64, kPathWide>(p, image, imageT, out, nslices, accumulate,
                                                      e, p.d_vlist + p.n_staged, p.n_gather,
-                                                     slots_per_view, wide_st);
+                                                     wide_st);
     if (wide_st != st) {
         int rc = check_hip(hipEventRecord(p.ev_join, wide_st), "dd forward: join");
         if (!rc) rc = check_hip(hipStreamWaitEvent(st, p.ev_join, 0), "dd forward: join");

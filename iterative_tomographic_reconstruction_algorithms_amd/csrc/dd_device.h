@@ -678,7 +678,7 @@ __device__ __forceinline__ void dd_fwd_block(const DdView* __restrict__ geom,
                                              const float* __restrict__ imageT,
                                              float* __restrict__ out, int64_t img_stride,
                                              int64_t sino_stride, int accumulate,
-                                             const EpiArgs& e, int bx, int view, int z, int blk)
+                                             const EpiArgs& e, int bx, int view, int z)
 {
     const int d = bx * B + threadIdx.x;
     const bool active = d < npix;
@@ -824,10 +824,13 @@ __device__ __forceinline__ void dd_fwd_block(const DdView* __restrict__ geom,
         }
         out[soff + o] = epi_forward<EPI>(acc, o, soff, z, e, &p0);
     }
-    if (epi_has_partials(EPI) && e.partials)
+    // one FP64 slot per 64-bin chunk of the view (the layout of dd_fwd_slots); a wave
+    // past the last bin has no chunk and writes nothing
+    const int chunk = bx * (B / 64) + (int)(threadIdx.x >> 6);
+    if (epi_has_partials(EPI) && e.partials && chunk * 64 < npix)
         wave_partials3(p0, 0.0, 0.0,
                        e.partials + ((int64_t)z * e.blocks_per_slice +
-                                     (int64_t)blk * (B / 64) + (threadIdx.x >> 6)) * 3);
+                                     (int64_t)view * ((npix + 63) / 64) + chunk) * 3);
 }
 
 // out[c][r] = in[r][c] for one 32x32 tile of one npix x npix slice, through LDS

@@ -148,16 +148,13 @@ constexpr int kDdBackTile = 64;             // default back tile (also the batch
 constexpr int kRdBlock = 64;
 constexpr int kEpiChunk = 256 * 8;          // pixels per elementwise-epilogue block
 
-inline size_t dd_fwd_blocks(int npix, int nang, int bins)
+// FP64 partial-sum slots of one forward slice: one per 64-bin chunk of a view (the
+// forward reduces per wave; a wave holds the chunk's 64 bins whatever the block width),
+// slot view * ceil(npix / 64) + chunk.  Every slot is written by exactly one wave, also
+// when launches of different block widths share the table.
+inline size_t dd_fwd_slots(int npix, int nang)
 {
-    return (size_t)((npix + bins - 1) / bins) * nang;
-}
-
-// FP64 partial-sum slots of one forward slice: one per wave (the forward reduces per
-// wave, not per block).
-inline size_t dd_fwd_slots(int npix, int nang, int bins)
-{
-    return dd_fwd_blocks(npix, nang, bins) * (size_t)(bins / 64);
+    return (size_t)((npix + 63) / 64) * nang;
 }
 
 inline size_t dd_back_blocks(int npix, int tile)
@@ -168,7 +165,7 @@ inline size_t dd_back_blocks(int npix, int tile)
 
 inline size_t fwd_blocks_per_slice(const Plan& p)
 {
-    if (p.projector == ITOMO_PROJ_DD) return dd_fwd_slots(p.npix, p.nang, kFwdBins[p.fwd_var]);
+    if (p.projector == ITOMO_PROJ_DD) return dd_fwd_slots(p.npix, p.nang);
     if (p.projector == ITOMO_PROJ_PD || p.projector == ITOMO_PROJ_SS)
         // partials come from the elementwise epilogue pass
         return ((size_t)p.nang * p.npix + kEpiChunk - 1) / kEpiChunk;
