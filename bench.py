@@ -513,10 +513,10 @@ def main():
     resid_rel = float(stats[0].sqrt()) / bnorm
 
     # the forward half: transpose for the column sweeps, the projection in two launches
-    # (views with at most 8 candidate pixels per bin and line, then the wider ones), the
-    # SIRT update pass
-    fwd_name = ("transpose + dd_fwd_kernel<FAST,STORE,%d> (main, wide) + fwd_epi_kernel<SIRT_FWD>"
-                % ([256, 128, 64][shapes["fwd"]]))
+    # (views with at most 8 candidate pixels per bin and line; the wider ones with 64-bin
+    # blocks on the plan's second stream at the same time), the SIRT update pass
+    fwd_name = ("transpose + dd_fwd_kernel<FAST,STORE,%d> (main) beside dd_fwd_kernel<FAST,STORE,64> "
+                "(wide) + fwd_epi_kernel<SIRT_FWD>" % ([256, 128, 64][shapes["fwd"]]))
     back_name = "dd_back_kernel<FAST,SIRT_BACK,%d,%d>" % tuple(
         [(64, 4), (32, 2), (32, 4)][shapes["back"]])
     if back_ms >= fwd_ms:
