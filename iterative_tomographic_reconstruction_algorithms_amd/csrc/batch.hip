@@ -3,9 +3,9 @@
 //
 // A batch holds members, each one distance-driven plan (npix, angles) with its own stack
 // of slices.  One iteration is two persistent launches (forward, back): a fixed grid of
-// 256-thread blocks pulls work items -- a 256-bin block of one view of one slice, or a
-// 64x64 tile of one slice -- from an atomic counter, largest items first, and runs the
-// same per-tile device code as the per-plan kernels.  Every output element is still
+// blocks pulls work items -- 64 bins of one view of one slice (one 64-thread block), or
+// a 64x64 tile of one slice (256 threads) -- from an atomic counter, largest items
+// first, and runs the same per-tile device code as the per-plan kernels.  Every output element is still
 // written by one thread in the reference's accumulation order, so results do not depend
 // on the schedule and equal the per-plan path bit for bit.
 #include <algorithm>
