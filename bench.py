@@ -38,6 +38,20 @@ def alg_bytes_back(n, v):
     return 4.0 * n * n * v + 4.0 * v * n + 3 * 4.0 * n * n
 
 
+def forward_half_name(fwd_variant):
+    """The forward half of a SIRT step at launch shape `fwd_variant`: the transpose for
+    the column sweeps, the projection in two launches (views with at most 8 candidate
+    pixels per bin and line; the wider ones with 64-bin blocks on the plan's second stream
+    at the same time), the SIRT update pass.  Also the key of its traffic entry."""
+    return ("transpose + dd_fwd_kernel<FAST,STORE,%d> (main) beside dd_fwd_kernel<FAST,STORE,64> "
+            "(wide) + fwd_epi_kernel<SIRT_FWD>" % ([256, 128, 64][fwd_variant]))
+
+
+def back_half_name(back_variant):
+    """The back half (projection with the fused SIRT update) at launch shape `back_variant`."""
+    return "dd_back_kernel<FAST,SIRT_BACK,%d,%d>" % tuple([(64, 4), (32, 2), (32, 4)][back_variant])
+
+
 def measured_traffic(kernel):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC summary
     (profiles/traffic_dd_sirt_2048x1800.json: FETCH_SIZE and WRITE_SIZE per dispatch,
@@ -512,13 +526,8 @@ def main():
     gups = 2.0 * N * N * V * K / secs / 1e9
     resid_rel = float(stats[0].sqrt()) / bnorm
 
-    # the forward half: transpose for the column sweeps, the projection in two launches
-    # (views with at most 8 candidate pixels per bin and line; the wider ones with 64-bin
-    # blocks on the plan's second stream at the same time), the SIRT update pass
-    fwd_name = ("transpose + dd_fwd_kernel<FAST,STORE,%d> (main) beside dd_fwd_kernel<FAST,STORE,64> "
-                "(wide) + fwd_epi_kernel<SIRT_FWD>" % ([256, 128, 64][shapes["fwd"]]))
-    back_name = "dd_back_kernel<FAST,SIRT_BACK,%d,%d>" % tuple(
-        [(64, 4), (32, 2), (32, 4)][shapes["back"]])
+    fwd_name = forward_half_name(shapes["fwd"])
+    back_name = back_half_name(shapes["back"])
     if back_ms >= fwd_ms:
         dom, dom_ms, dom_bytes = back_name, back_ms, alg_bytes_back(N, V)
     else:
